@@ -1,0 +1,50 @@
+// Shared host/device definitions of the kNN selection engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace snn {
+
+// Sentinel of unfilled neighbour slots and initial minima: the reference's
+// MAX_FLOAT_TYPE (speedynn/lib/types.h:9) rounded to float, i.e. FLT_MAX.
+constexpr float kMaxF = 3.402823466e+38f;
+
+// Loss codes, speedynn/lib/loss_functions.h:11-19.
+enum Loss : int {
+  kRmse = 0, kRmseNorm = 1, kMad = 2, kMadNorm = 3, kZeroOne = 4, kStd = 5,
+  kRmseNormMadNormStd = 6, kRmseNormMadNormGalaxies = 7, kRmseStd = 8
+};
+
+// Device-resident selection state.  Every per-step kernel reads it, so a step
+// needs no host round trip and can be captured in a graph.
+struct SelState {
+  int step;       // completed steps
+  int stopped;    // auto-stop fired or no candidate left
+  int ncand;      // candidates of the current step
+  int fstar;      // feature chosen by the last completed step
+  int best_k;     // its optimal k
+  int num_final;  // features after the last completed step
+  int error;      // 1: no candidate had a finite error
+  int pos;        // record position of the next step
+  int pending;    // fstar chosen but not yet applied to D
+  int fslot;      // candidate slot of fstar in the step that chose it
+  int pad[2];
+};
+
+// Fixed description of one selection problem on the device.
+struct Problem {
+  int64_t n;      // points
+  int64_t npf;    // points per fold = n / k_fold
+  int d;          // features
+  int cpad;       // candidate slots, multiple of 64
+  int nk;         // number of k values
+  int nl;         // number of losses
+  int kmax;       // max k value
+  int method;     // 1 forward, 0 backward
+  int model_type; // 0 regression, 1 classification
+  int auto_feature;
+  int max_steps;  // steps to run at most
+  int num_desired;
+};
+
+}  // namespace snn

@@ -1,0 +1,530 @@
+// Host side of the kNN wrapper feature-selection engine: the C ABI of
+// include/speedynn.h over HIP.  One selection runs as a fixed sequence of
+// kernels per step on one stream (gather candidates, search, losses, arg-min,
+// update); every step reads the device-resident state, so the host issues all
+// steps without synchronising and reads the results once at the end.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/speedynn.h"
+#include "common.h"
+#include "kernels.h"
+
+using snn::Problem;
+using snn::SelState;
+
+namespace {
+
+thread_local std::string g_err = "no error";
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(e_ == hipErrorOutOfMemory ? SNN_ENOMEM : SNN_EDEVICE,                     \
+                  std::string(#expr) + " failed: " + hipGetErrorString(e_));               \
+  } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t want) {
+    if (want <= bytes && p) return hipSuccess;
+    if (p) {
+      hipFree(p);
+      p = nullptr;
+      bytes = 0;
+    }
+    if (want == 0) want = 256;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) bytes = want;
+    return e;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace
+
+struct snn_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevBuf X, y, XT, D, Dlo, Xc, P, err, val, bestk, cand, sel, ordering, errors, cand_err, kvals,
+      losses, state, scratch, NN, seeds, visited;
+};
+
+namespace {
+
+int validate_losses(const int* loss_functions, int num_loss) {
+  if (num_loss < 1) return fail(SNN_EINVAL, "at least one loss function is required");
+  for (int l = 0; l < num_loss; ++l) {
+    const int c = loss_functions[l];
+    if (c < 0 || c > 8) return fail(SNN_EINVAL, "unknown loss function code " + std::to_string(c));
+    if (c == snn::kMad || c == snn::kMadNorm || c == snn::kRmseNormMadNormStd ||
+        c == snn::kRmseNormMadNormGalaxies)
+      return fail(SNN_ENOTSUP, "median-based losses (mad, mad_norm and their composites) are "
+                               "not implemented on the device yet");
+  }
+  return SNN_OK;
+}
+
+struct Timing {
+  std::vector<hipEvent_t> ev;
+  hipEvent_t make() {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ev.push_back(e);
+    return e;
+  }
+  ~Timing() {
+    for (auto e : ev) hipEventDestroy(e);
+  }
+};
+
+double elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (a && b && hipEventElapsedTime(&ms, a, b) == hipSuccess) return ms;
+  return 0.0;
+}
+
+int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy,
+                const int* k_vals, int num_k, const int* loss_functions, int num_loss,
+                const snn_params* prm, int* selected_dimensions, int* selected_ordering,
+                float* selected_errors, float* candidate_errors, snn_summary* summary) {
+  if (!ctx) return fail(SNN_EINVAL, "null context");
+  if (!prm) return fail(SNN_EINVAL, "null params");
+  if (n < 2 || d < 1) return fail(SNN_EINVAL, "need n >= 2 points and d >= 1 features");
+  if (n > (int64_t)INT32_MAX) return fail(SNN_EINVAL, "n exceeds the 32-bit neighbour index");
+  if (prm->method != 0 && prm->method != 1) return fail(SNN_EINVAL, "method must be 0 or 1");
+  if (prm->model_type != 0 && prm->model_type != 1)
+    return fail(SNN_EINVAL, "model_type must be 0 (regression) or 1 (classification)");
+  if (prm->k_fold < 1 || prm->k_fold > n)
+    return fail(SNN_EINVAL, "k_fold must be in [1, n] (k_fold > n divides by zero in the reference)");
+  if (num_k < 1 || !k_vals) return fail(SNN_EINVAL, "at least one k value is required");
+  int kmax = 0;
+  for (int m = 0; m < num_k; ++m) {
+    if (k_vals[m] < 1) return fail(SNN_EINVAL, "k values must be >= 1");
+    kmax = k_vals[m] > kmax ? k_vals[m] : kmax;
+  }
+  if (snn::search_list_size(kmax) < 0) return fail(SNN_ENOTSUP, "k values above 64 are not supported");
+  int rc = validate_losses(loss_functions, num_loss);
+  if (rc) return rc;
+  if (!prm->auto_feature && prm->num_features < 1)
+    return fail(SNN_EINVAL, "num_features must be > 0");
+  if (!prm->auto_feature && prm->num_features > d)
+    return fail(SNN_EINVAL, "num_features must be <= the number of features");
+
+  Problem pb{};
+  pb.n = n;
+  pb.npf = n / prm->k_fold;
+  pb.d = d;
+  pb.cpad = (d + 63) / 64 * 64;
+  pb.nk = num_k;
+  pb.nl = num_loss;
+  pb.kmax = kmax;
+  pb.method = prm->method;
+  pb.model_type = prm->model_type;
+  pb.auto_feature = prm->auto_feature ? 1 : 0;
+  const bool fwd = prm->method == 1;
+  pb.num_desired = prm->auto_feature ? (fwd ? d : 0) : prm->num_features;
+  pb.max_steps = fwd ? pb.num_desired : d - pb.num_desired;
+  const bool comp = prm->form == SNN_FORM_PRECISION;
+
+  HIPCHK(hipSetDevice(ctx->device));
+  const size_t nn = (size_t)n * (size_t)n;
+  HIPCHK(ctx->XT.ensure(sizeof(float) * (size_t)n * d));
+  HIPCHK(ctx->D.ensure(sizeof(float) * nn));
+  if (comp) HIPCHK(ctx->Dlo.ensure(sizeof(float) * nn));
+  HIPCHK(ctx->Xc.ensure(sizeof(float) * (size_t)n * pb.cpad));
+  HIPCHK(ctx->P.ensure(sizeof(float) * (size_t)num_k * pb.cpad * n));
+  HIPCHK(ctx->err.ensure(sizeof(float) * (size_t)num_loss * num_k * pb.cpad));
+  HIPCHK(ctx->val.ensure(sizeof(float) * (size_t)num_loss * pb.cpad));
+  HIPCHK(ctx->bestk.ensure(sizeof(int) * pb.cpad));
+  HIPCHK(ctx->cand.ensure(sizeof(int) * d));
+  HIPCHK(ctx->sel.ensure(sizeof(int) * d));
+  HIPCHK(ctx->ordering.ensure(sizeof(int) * d));
+  HIPCHK(ctx->errors.ensure(sizeof(float) * (size_t)num_loss * d));
+  HIPCHK(ctx->kvals.ensure(sizeof(int) * num_k));
+  HIPCHK(ctx->losses.ensure(sizeof(int) * num_loss));
+  HIPCHK(ctx->state.ensure(sizeof(SelState)));
+  // precision form, forward: exact pruning seeded by the chosen candidate's
+  // lists of the previous step
+  const bool pruned = comp && fwd;
+  const int K = snn::search_list_size(kmax);
+  if (pruned) {
+    HIPCHK(ctx->NN.ensure(sizeof(int) * (size_t)n * pb.cpad * K));
+    HIPCHK(ctx->seeds.ensure(sizeof(int) * (size_t)n * K));
+    HIPCHK(ctx->visited.ensure(sizeof(unsigned long long)));
+    HIPCHK(hipMemsetAsync(ctx->visited.p, 0, sizeof(unsigned long long), ctx->stream));
+  }
+  float* cand_err = nullptr;
+  if (candidate_errors) {
+    HIPCHK(ctx->cand_err.ensure(sizeof(float) * (size_t)d * d));
+    cand_err = ctx->cand_err.as<float>();
+  }
+
+  hipStream_t s = ctx->stream;
+  HIPCHK(hipMemcpyAsync(ctx->kvals.p, k_vals, sizeof(int) * num_k, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(ctx->losses.p, loss_functions, sizeof(int) * num_loss,
+                        hipMemcpyHostToDevice, s));
+  if (cand_err) {
+    HIPCHK(snn::launch_fill(cand_err, (int64_t)d * d, std::nanf(""), s));
+  }
+
+  Timing tm;
+  hipEvent_t e0 = tm.make(), e1 = tm.make();
+  HIPCHK(hipEventRecord(e0, s));
+  SelState* st = ctx->state.as<SelState>();
+  float* D = ctx->D.as<float>();
+  float* Dlo = comp ? ctx->Dlo.as<float>() : nullptr;
+  HIPCHK(snn::launch_transpose(dX, ctx->XT.as<float>(), n, d, s));
+  HIPCHK(snn::launch_init_state(st, pb, ctx->cand.as<int>(), ctx->sel.as<int>(),
+                                ctx->ordering.as<int>(), ctx->errors.as<float>(), s));
+  if (fwd) {
+    HIPCHK(snn::launch_fill(D, (int64_t)nn, 0.0f, s));
+    if (Dlo) HIPCHK(snn::launch_fill(Dlo, (int64_t)nn, 0.0f, s));
+  } else {
+    HIPCHK(snn::launch_gram_exact(ctx->XT.as<float>(), D, Dlo, n, d, s));
+  }
+  HIPCHK(hipEventRecord(e1, s));
+
+  snn::SearchLaunch sl{};
+  sl.D = D;
+  sl.Xc = ctx->Xc.as<float>();
+  sl.y = dy;
+  sl.kvals = ctx->kvals.as<int>();
+  sl.st = st;
+  sl.P = ctx->P.as<float>();
+  sl.NN = nullptr;
+  sl.n = n;
+  sl.npf = pb.npf;
+  sl.cpad = pb.cpad;
+  sl.nk = num_k;
+  sl.kmax = kmax;
+  sl.mult = fwd ? 1 : -1;
+  sl.model_type = prm->model_type;
+  sl.zero = 0;
+  if (pruned) {
+    sl.NN = ctx->NN.as<int>();
+    sl.seeds = ctx->seeds.as<int>();
+    sl.visited = ctx->visited.as<unsigned long long>();
+  }
+
+  // per step: [a] gather [b] search [c] losses + arg-min [u] update [next a]
+  std::vector<hipEvent_t> marks;
+  for (int step = 0; step < pb.max_steps; ++step) {
+    hipEvent_t a = tm.make(), b = tm.make(), c = tm.make(), u = tm.make();
+    HIPCHK(hipEventRecord(a, s));
+    HIPCHK(snn::launch_compact(dX, ctx->cand.as<int>(), st, ctx->Xc.as<float>(), n, d, pb.cpad, s));
+    HIPCHK(hipEventRecord(b, s));
+    if (pruned && step > 0) {
+      HIPCHK(snn::launch_search_pruned(sl, s));
+    } else {
+      sl.zero = pruned ? 1 : 0;  // first forward step: D is 0
+      HIPCHK(snn::launch_search_dense(sl, s));
+    }
+    HIPCHK(hipEventRecord(c, s));
+    HIPCHK(snn::launch_loss(ctx->P.as<float>(), dy, ctx->losses.as<int>(), st,
+                            ctx->err.as<float>(), n, pb.cpad, num_k, num_loss, d, s));
+    HIPCHK(snn::launch_select(st, pb, ctx->kvals.as<int>(), ctx->err.as<float>(),
+                              ctx->cand.as<int>(), ctx->sel.as<int>(), ctx->ordering.as<int>(),
+                              ctx->errors.as<float>(), ctx->val.as<float>(), ctx->bestk.as<int>(),
+                              cand_err, s));
+    if (pruned)
+      HIPCHK(snn::launch_seeds(ctx->NN.as<int>(), st, ctx->seeds.as<int>(), n, pb.cpad, kmax, s));
+    HIPCHK(hipEventRecord(u, s));
+    HIPCHK(snn::launch_update(D, Dlo, ctx->XT.as<float>(), n, st, -1, fwd ? 1 : -1, s));
+    marks.push_back(a);
+    marks.push_back(b);
+    marks.push_back(c);
+    marks.push_back(u);
+  }
+  hipEvent_t e2 = tm.make();
+  HIPCHK(hipEventRecord(e2, s));
+  HIPCHK(hipStreamSynchronize(s));
+
+  SelState hs{};
+  HIPCHK(hipMemcpy(&hs, st, sizeof(SelState), hipMemcpyDeviceToHost));
+  if (selected_dimensions)
+    HIPCHK(hipMemcpy(selected_dimensions, ctx->sel.p, sizeof(int) * d, hipMemcpyDeviceToHost));
+  if (selected_ordering)
+    HIPCHK(hipMemcpy(selected_ordering, ctx->ordering.p, sizeof(int) * d, hipMemcpyDeviceToHost));
+  if (selected_errors)
+    HIPCHK(hipMemcpy(selected_errors, ctx->errors.p, sizeof(float) * (size_t)num_loss * d,
+                     hipMemcpyDeviceToHost));
+  if (candidate_errors)
+    HIPCHK(hipMemcpy(candidate_errors, cand_err, sizeof(float) * (size_t)d * d,
+                     hipMemcpyDeviceToHost));
+
+  if (summary) {
+    std::memset(summary, 0, sizeof(*summary));
+    summary->num_final = hs.num_final;
+    summary->optimal_k = hs.best_k;
+    summary->steps = hs.step;
+    summary->ms_total = elapsed(e0, e2);
+    summary->ms_distance = elapsed(e0, e1);
+    double ms_search = 0, ms_update = 0, ms_tally = 0;
+    for (size_t i = 0; i + 3 < marks.size(); i += 4) {
+      ms_tally += elapsed(marks[i], marks[i + 1]) + elapsed(marks[i + 2], marks[i + 3]);
+      ms_search += elapsed(marks[i + 1], marks[i + 2]);
+      const hipEvent_t next = (i + 4 < marks.size()) ? marks[i + 4] : e2;
+      ms_update += elapsed(marks[i + 3], next);
+    }
+    summary->ms_search = ms_search;
+    summary->ms_update = ms_update;
+    summary->ms_tally = ms_tally;
+    // candidates evaluated per step: fwd d, d-1, ...; bwd d, d-1, ...
+    int64_t ev = 0, pruned_full = 0;
+    for (int t = 0; t < hs.step; ++t) {
+      ev += (int64_t)(d - t) * n;
+      if (pruned && t > 0) pruned_full += (int64_t)(d - t) * n * (n - pb.npf);
+    }
+    summary->evals = ev;
+    if (pruned) {
+      unsigned long long done = 0;
+      HIPCHK(hipMemcpy(&done, ctx->visited.p, sizeof(done), hipMemcpyDeviceToHost));
+      summary->pruned_point_evals = pruned_full - (int64_t)done;
+    }
+  }
+  if (hs.error) return fail(SNN_ESTATE, "no candidate had a finite validation error");
+  return SNN_OK;
+}
+
+std::mutex g_default_mu;
+snn_ctx* g_default = nullptr;
+int g_default_form = SNN_FORM_PRECISION;
+std::vector<int> g_sel, g_order;
+std::vector<float> g_errors;
+int g_best_k = 0;
+
+snn_ctx* default_ctx() {
+  if (!g_default) {
+    int dev = 0;
+    if (const char* e = std::getenv("SPEEDYNN_DEVICE")) dev = std::atoi(e);
+    if (snn_create(&g_default, dev) != SNN_OK) g_default = nullptr;
+  }
+  return g_default;
+}
+
+int train_common(int method, float* X, int n, int d, float* Y, int nY, int* kvals, int nk,
+                 int num_features, int kfold, int* losses, int nl, int auto_feature,
+                 int model_type) {
+  std::lock_guard<std::mutex> lock(g_default_mu);
+  if (nY != n) return fail(SNN_EINVAL, "Ytrain must have one label per row of Xtrain");
+  snn_ctx* ctx = default_ctx();
+  if (!ctx) return SNN_EDEVICE;
+  snn_params p{};
+  p.method = method;
+  p.model_type = model_type;
+  p.k_fold = kfold;
+  p.num_features = num_features;
+  p.auto_feature = auto_feature;
+  p.form = g_default_form;
+  std::vector<int> sel(d), order(d);
+  std::vector<float> errs((size_t)nl * d);
+  snn_summary sm{};
+  int rc = snn_select(ctx, X, n, d, Y, kvals, nk, losses, nl, &p, sel.data(), order.data(),
+                      errs.data(), nullptr, &sm);
+  if (rc != SNN_OK) return rc;
+  g_sel.swap(sel);
+  g_order.swap(order);
+  g_errors.swap(errs);
+  g_best_k = sm.optimal_k;
+  return sm.num_final;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* snn_last_error(void) { return g_err.c_str(); }
+
+void snn_set_default_form(int form) { g_default_form = form; }
+
+int snn_create(snn_ctx** out, int device) {
+  if (!out) return fail(SNN_EINVAL, "null output pointer");
+  *out = nullptr;
+  int count = 0;
+  HIPCHK(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count)
+    return fail(SNN_EDEVICE, "HIP device " + std::to_string(device) + " not present (" +
+                                 std::to_string(count) + " visible)");
+  HIPCHK(hipSetDevice(device));
+  snn_ctx* c = new snn_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(SNN_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return SNN_OK;
+}
+
+void snn_destroy(snn_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  DevBuf* bufs[] = {&c->X, &c->y, &c->XT, &c->D, &c->Dlo, &c->Xc, &c->P, &c->err, &c->val,
+                    &c->bestk, &c->cand, &c->sel, &c->ordering, &c->errors, &c->cand_err,
+                    &c->kvals, &c->losses, &c->state, &c->scratch};
+  for (DevBuf* b : bufs) b->release();
+  hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int snn_select_device(snn_ctx* ctx, const float* X_device, int64_t n, int d,
+                      const float* y_device, const int* k_vals, int num_k,
+                      const int* loss_functions, int num_loss, const snn_params* params,
+                      int* selected_dimensions, int* selected_ordering, float* selected_errors,
+                      float* candidate_errors, snn_summary* summary) {
+  if (!X_device || !y_device) return fail(SNN_EINVAL, "null input");
+  return select_impl(ctx, X_device, n, d, y_device, k_vals, num_k, loss_functions, num_loss,
+                     params, selected_dimensions, selected_ordering, selected_errors,
+                     candidate_errors, summary);
+}
+
+int snn_select(snn_ctx* ctx, const float* X, int64_t n, int d, const float* y,
+               const int* k_vals, int num_k, const int* loss_functions, int num_loss,
+               const snn_params* params, int* selected_dimensions, int* selected_ordering,
+               float* selected_errors, float* candidate_errors, snn_summary* summary) {
+  if (!ctx) return fail(SNN_EINVAL, "null context");
+  if (!X || !y) return fail(SNN_EINVAL, "null input");
+  if (n < 1 || d < 1) return fail(SNN_EINVAL, "empty input");
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(ctx->X.ensure(sizeof(float) * (size_t)n * d));
+  HIPCHK(ctx->y.ensure(sizeof(float) * (size_t)n));
+  HIPCHK(hipMemcpyAsync(ctx->X.p, X, sizeof(float) * (size_t)n * d, hipMemcpyHostToDevice,
+                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->y.p, y, sizeof(float) * (size_t)n, hipMemcpyHostToDevice,
+                        ctx->stream));
+  return select_impl(ctx, ctx->X.as<float>(), n, d, ctx->y.as<float>(), k_vals, num_k,
+                     loss_functions, num_loss, params, selected_dimensions, selected_ordering,
+                     selected_errors, candidate_errors, summary);
+}
+
+int snn_distance_rows(snn_ctx* ctx, const float* X, int64_t n, int d, int method,
+                      const int* order, int s, int form, const int64_t* rows, int nrows,
+                      float* rows_out, float* lo_out) {
+  if (!ctx || !X || (!rows && nrows > 0) || !rows_out) return fail(SNN_EINVAL, "null argument");
+  if (n < 1 || d < 1 || s < 0 || s > d) return fail(SNN_EINVAL, "bad sizes");
+  for (int t = 0; t < s; ++t)
+    if (order[t] < 0 || order[t] >= d) return fail(SNN_EINVAL, "feature index out of range");
+  for (int r = 0; r < nrows; ++r)
+    if (rows[r] < 0 || rows[r] >= n) return fail(SNN_EINVAL, "row index out of range");
+  const bool comp = form == SNN_FORM_PRECISION;
+  HIPCHK(hipSetDevice(ctx->device));
+  const size_t nn = (size_t)n * (size_t)n;
+  HIPCHK(ctx->X.ensure(sizeof(float) * (size_t)n * d));
+  HIPCHK(ctx->XT.ensure(sizeof(float) * (size_t)n * d));
+  HIPCHK(ctx->D.ensure(sizeof(float) * nn));
+  if (comp) HIPCHK(ctx->Dlo.ensure(sizeof(float) * nn));
+  hipStream_t st = ctx->stream;
+  float* D = ctx->D.as<float>();
+  float* Dlo = comp ? ctx->Dlo.as<float>() : nullptr;
+  HIPCHK(hipMemcpyAsync(ctx->X.p, X, sizeof(float) * (size_t)n * d, hipMemcpyHostToDevice, st));
+  HIPCHK(snn::launch_transpose(ctx->X.as<float>(), ctx->XT.as<float>(), n, d, st));
+  if (method == 1) {
+    HIPCHK(snn::launch_fill(D, (int64_t)nn, 0.0f, st));
+    if (Dlo) HIPCHK(snn::launch_fill(Dlo, (int64_t)nn, 0.0f, st));
+  } else {
+    HIPCHK(snn::launch_gram_exact(ctx->XT.as<float>(), D, Dlo, n, d, st));
+  }
+  for (int t = 0; t < s; ++t)
+    HIPCHK(snn::launch_update(D, Dlo, ctx->XT.as<float>(), n, nullptr, order[t],
+                              method == 1 ? 1 : -1, st));
+  for (int r = 0; r < nrows; ++r) {
+    HIPCHK(hipMemcpyAsync(rows_out + (size_t)r * n, D + (size_t)rows[r] * n, sizeof(float) * n,
+                          hipMemcpyDeviceToHost, st));
+    if (lo_out && Dlo)
+      HIPCHK(hipMemcpyAsync(lo_out + (size_t)r * n, Dlo + (size_t)rows[r] * n, sizeof(float) * n,
+                            hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  return SNN_OK;
+}
+
+int snn_loss(snn_ctx* ctx, const float* labels, const float* predictions, int64_t n,
+             int loss_function, float* out) {
+  if (!ctx || !labels || !predictions || !out) return fail(SNN_EINVAL, "null argument");
+  if (n < 1) return fail(SNN_EINVAL, "empty input");
+  int rc = validate_losses(&loss_function, 1);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(ctx->scratch.ensure(sizeof(float) * (size_t)(2 * n + 64)));
+  float* dy = ctx->scratch.as<float>();
+  float* dp = dy + n;
+  float* dout = dp + n;
+  HIPCHK(hipMemcpyAsync(dy, labels, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(dp, predictions, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(snn::launch_loss_single(dy, dp, n, loss_function, dout, ctx->stream));
+  HIPCHK(hipMemcpyAsync(out, dout, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return SNN_OK;
+}
+
+int train_backward_selection(float* Xtrain, int nXtrain, int dXtrain, float* Ytrain, int nYtrain,
+                             int* kvals, int num_kvals, int num_features, int kfold,
+                             int* loss_functions, int num_loss_functions, int auto_feature,
+                             int model_type) {
+  return train_common(0, Xtrain, nXtrain, dXtrain, Ytrain, nYtrain, kvals, num_kvals,
+                      num_features, kfold, loss_functions, num_loss_functions, auto_feature,
+                      model_type);
+}
+
+int train_forward_selection(float* Xtrain, int nXtrain, int dXtrain, float* Ytrain, int nYtrain,
+                            int* kvals, int num_kvals, int num_features, int kfold,
+                            int* loss_functions, int num_loss_functions, int auto_feature,
+                            int model_type) {
+  return train_common(1, Xtrain, nXtrain, dXtrain, Ytrain, nYtrain, kvals, num_kvals,
+                      num_features, kfold, loss_functions, num_loss_functions, auto_feature,
+                      model_type);
+}
+
+void get_selected_dimensions(int* out, int num) {
+  for (int i = 0; i < num; ++i) out[i] = i < (int)g_sel.size() ? g_sel[i] : 0;
+}
+
+void get_selected_ordering(int* out, int num) {
+  for (int i = 0; i < num; ++i) out[i] = i < (int)g_order.size() ? g_order[i] : 0;
+}
+
+void get_selected_errors(float* out, int num) {
+  for (int i = 0; i < num; ++i) out[i] = i < (int)g_errors.size() ? g_errors[i] : 0.0f;
+}
+
+int get_final_optimal_model_k(void) { return g_best_k; }
+
+float get_loss_all(float* labels, int num_labels, float* predictions, int num_predictions,
+                   int loss_function) {
+  std::lock_guard<std::mutex> lock(g_default_mu);
+  if (num_labels != num_predictions) {
+    fail(SNN_EINVAL, "labels and predictions differ in length");
+    return std::nanf("");
+  }
+  snn_ctx* ctx = default_ctx();
+  float out = std::nanf("");
+  if (!ctx || snn_loss(ctx, labels, predictions, num_labels, loss_function, &out) != SNN_OK)
+    return std::nanf("");
+  return out;
+}
+
+}  // extern "C"

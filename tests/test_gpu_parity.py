@@ -1,0 +1,196 @@
+"""GPU parity: the HIP engine against the reference's golden vectors and the
+CPU oracle, through the C ABI.  Integer and index results must be bit-exact,
+and so must every float error (both sides compute the same float32 ops in the
+same order)."""
+import base64
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, assert_same_result, expected_arrays, golden_paths, load_case
+from oracle import oracle as O
+from speedynn_amd import _lib, datasets as ds
+
+pytestmark = pytest.mark.gpu
+
+LOSS = _lib  # noqa: F841  (module used for codes below)
+LOSS_CODES = {"rmse": 0, "rmse_norm": 1, "mad": 2, "mad_norm": 3, "zero_one": 4, "std": 5,
+              "rmse_norm_mad_norm_std": 6, "rmse_norm_mad_norm_galaxies": 7, "rmse_std": 8}
+MEDIAN_LOSSES = {"mad", "mad_norm", "rmse_norm_mad_norm_std", "rmse_norm_mad_norm_galaxies"}
+
+
+@pytest.fixture(scope="module")
+def engine():
+    print("engine: creating context on device 0", flush=True)
+    return _lib.Engine(0)
+
+
+def run(engine, X, y, inp, form, candidate_errors=False):
+    codes = [LOSS_CODES[l] for l in inp["loss_functions"]]
+    return engine.select(X, y, inp["k_vals"], codes,
+                         method=1 if inp["method"] == "forward" else 0,
+                         model_type=1 if inp["model_type"] == "classification" else 0,
+                         k_fold=inp["k_fold"], num_features=inp["num_features"],
+                         auto_feature=inp["auto_feature"], form=form,
+                         candidate_errors=candidate_errors)
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["plain", "precision"])
+@pytest.mark.parametrize("path", golden_paths(), ids=os.path.basename)
+def test_engine_matches_reference_golden(engine, path, form):
+    rec, X, y = load_case(path)
+    inp = rec["inputs"]
+    print(f"case {os.path.basename(path)} n={X.shape[0]} d={X.shape[1]} form={form}", flush=True)
+    if MEDIAN_LOSSES & set(inp["loss_functions"]):
+        with pytest.raises(_lib.EngineError):
+            run(engine, X, y, inp, form)
+        pytest.skip("median-based losses are not implemented on the device yet")
+    got = run(engine, X, y, inp, form)
+    assert_same_result(got, expected_arrays(rec), os.path.basename(path))
+
+
+RANDOM = [
+    # seed, n, d, k_vals, k_fold, model, method, num_features
+    (61, 300, 12, [1, 3, 5], 300, "classification", "forward", 5),
+    (62, 257, 9, [2, 4, 8], 13, "classification", "backward", 4),
+    (63, 400, 10, [5, 10, 20], 400, "regression", "forward", 4),
+    (64, 333, 8, [15], 333, "regression", "backward", 3),
+    (65, 130, 70, [7, 3], 130, "classification", "forward", 3),
+    (66, 190, 6, [32, 50], 190, "regression", "forward", 3),
+]
+
+
+def _random_case(seed, n, d, model):
+    if model == "classification":
+        X, y, _ = ds.classification_argmax(seed, n, d, 3, 3, 0.1)
+        if seed % 2:
+            X = ds.quantized(X, 0.25)
+    else:
+        X, y, _ = ds.regression_sum(seed, n, d, 3, 0.5)
+    return X, y
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["plain", "precision"])
+@pytest.mark.parametrize("case", RANDOM, ids=lambda c: f"seed{c[0]}")
+def test_every_candidate_error_matches_oracle(engine, case, form):
+    seed, n, d, k_vals, k_fold, model, method, nf = case
+    X, y = _random_case(seed, n, d, model)
+    loss = "zero_one" if model == "classification" else "rmse"
+    print(f"random case seed={seed} n={n} d={d} form={form}", flush=True)
+    o = O.select(X, y, k_vals, nf, k_fold, [loss], False, model, method, record_candidates=True)
+    inp = dict(k_vals=k_vals, loss_functions=[loss], method=method, model_type=model,
+               k_fold=k_fold, num_features=nf, auto_feature=False)
+    g = run(engine, X, y, inp, form, candidate_errors=True)
+    assert_same_result(g, o)
+    steps = g["steps"]
+    np.testing.assert_array_equal(g["candidate_errors"][:steps].view(np.uint32),
+                                  o["candidate_errors"][:steps].view(np.uint32))
+
+
+@pytest.mark.parametrize("method", ["forward", "backward"])
+def test_distance_rows_bit_exact(engine, method):
+    X, _, _ = ds.regression_sum(70, 500, 40, 3, 0.5)
+    order = [7, 3, 33, 0, 19]
+    rows = [0, 1, 250, 499]
+    got, _ = engine.distance_rows(X, 1 if method == "forward" else 0, order, rows, form=0)
+    for r, q in enumerate(rows):
+        np.testing.assert_array_equal(got[r].view(np.uint32),
+                                      O.distance_row(X, q, method, order).view(np.uint32))
+
+
+def test_compensated_rows_keep_plain_bits_and_gain_accuracy(engine):
+    X, _, _ = ds.regression_sum(71, 400, 64, 3, 0.5)
+    X = X * np.float32(37.0)
+    order = list(range(0, 64, 2))
+    rows = [3, 200]
+    plain, _ = engine.distance_rows(X, 1, order, rows, form=0)
+    hi, lo = engine.distance_rows(X, 1, order, rows, form=1)
+    np.testing.assert_array_equal(plain.view(np.uint32), hi.view(np.uint32))
+    for r, q in enumerate(rows):
+        exact = O.distance_row_f64(X, q, order)
+        nz = exact > 0
+        e_plain = np.abs(plain[r][nz] - exact[nz]) / exact[nz]
+        e_comp = np.abs(hi[r][nz].astype(np.float64) + lo[r][nz] - exact[nz]) / exact[nz]
+        assert e_comp.max() < 1e-9
+        assert e_comp.max() < e_plain.max()
+
+
+def test_backward_full_matrix_relative_error(engine):
+    X, _, _ = ds.regression_sum(72, 300, 256, 3, 0.5)
+    got, lo = engine.distance_rows(X, 0, [], [0, 150, 299], form=1)
+    for r, q in enumerate([0, 150, 299]):
+        exact = O.distance_row_f64(X, q, list(range(256)))
+        nz = exact > 0
+        rel = np.abs(got[r][nz] - exact[nz]) / exact[nz]
+        assert rel.max() < 1e-5
+        rel2 = np.abs(got[r][nz].astype(np.float64) + lo[r][nz] - exact[nz]) / exact[nz]
+        assert rel2.max() < 1e-11
+
+
+def test_losses_match_reference_known_answers(engine):
+    with open(os.path.join(GOLDEN, "losses_votes.json")) as fh:
+        rec = json.load(fh)
+    sets = {k: [np.frombuffer(base64.b64decode(s), np.float32) for s in v]
+            for k, v in rec["sets"].items()}
+    for item in rec["losses"]:
+        if item["loss"] in MEDIAN_LOSSES:
+            continue
+        y, p = sets[item["set"]]
+        got = np.float32(engine.loss(y, p, LOSS_CODES[item["loss"]])).view(np.uint32)
+        assert int(got) == item["value_bits"], item
+
+
+def test_dropin_api_and_swig_surface_on_c0():
+    from speedynn_amd import IncrementalNNFeatureSelection
+    from speedynn_amd import nearest_neighbors_feature_selection_gpu as M
+    rec, X, y = load_case(os.path.join(GOLDEN, "c0.json"))
+    exp = expected_arrays(rec)
+    m = IncrementalNNFeatureSelection(X, y, k_vals=[10], num_features=10, k_fold=2000,
+                                      method="forward", loss_functions=["zero_one"],
+                                      model_type="classification")
+    m.train()
+    np.testing.assert_array_equal(m.get_selected_ordering(), exp["selected_ordering"])
+    np.testing.assert_array_equal(m.get_selected_errors()[:, 0].view(np.uint32),
+                                  exp["selected_errors"].view(np.uint32))
+    nf = M.train_forward_selection(X, y, np.array([10], np.int32), 10, 2000,
+                                   np.array([4], np.int32), 0, 1)
+    assert nf == exp["num_final"]
+    order = np.zeros(50, np.int32)
+    M.get_selected_ordering(order)
+    np.testing.assert_array_equal(order, exp["selected_ordering"])
+    assert M.get_final_optimal_model_k() == exp["optimal_k"]
+
+
+def test_invalid_arguments_return_errors(engine):
+    X, y, _ = ds.classification_sign(80, 50, 4, 2, 0.0)
+    with pytest.raises(_lib.EngineError):
+        engine.select(X, y, [3], [4], method=1, model_type=1, k_fold=51, num_features=2)
+    with pytest.raises(_lib.EngineError):
+        engine.select(X, y, [0], [4], method=1, model_type=1, k_fold=50, num_features=2)
+    with pytest.raises(_lib.EngineError):
+        engine.select(X, y, [3], [9], method=1, model_type=1, k_fold=50, num_features=2)
+    with pytest.raises(_lib.EngineError):
+        engine.select(X, y, [3], [4], method=1, model_type=1, k_fold=50, num_features=5)
+    # the context stays usable after rejected calls
+    r = engine.select(X, y, [3], [4], method=1, model_type=1, k_fold=50, num_features=2)
+    assert r["num_final"] == 2
+
+
+C1 = os.path.join(GOLDEN, "c1_oracle.json")
+
+
+@pytest.mark.slow
+@pytest.mark.skipif(not os.path.exists(C1), reason="c1 oracle fixture not generated")
+@pytest.mark.parametrize("form", [0, 1], ids=["plain", "precision"])
+def test_c1_full_size_matches_oracle(engine, form):
+    """n=20000, d=200, k=10, 3 classes, forward to 20: the selected sequence, the
+    per-step errors (integer LOO counts / n) and all 3810 candidate errors."""
+    rec, X, y = load_case(C1)
+    print(f"c1 form={form}", flush=True)
+    got = run(engine, X, y, rec["inputs"], form, candidate_errors=True)
+    assert_same_result(got, expected_arrays(rec), "c1")
+    steps = got["steps"]
+    exp_c = np.array(rec["expected"]["candidate_errors_bits"], np.uint32).reshape(steps, -1)
+    np.testing.assert_array_equal(got["candidate_errors"][:steps].view(np.uint32), exp_c)
