@@ -47,4 +47,11 @@ struct Problem {
   int num_desired;
 };
 
+// Wave-uniform value (SGPR) of a value known to be uniform.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ float lane_value(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
 }  // namespace snn

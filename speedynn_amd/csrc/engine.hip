@@ -68,7 +68,7 @@ struct snn_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   DevBuf X, y, XT, D, Dlo, Xc, P, err, val, bestk, cand, sel, ordering, errors, cand_err, kvals,
-      losses, state, scratch, NN, seeds, visited;
+      losses, state, scratch, NN, seeds, visited, perm, key;
 };
 
 namespace {
@@ -165,13 +165,12 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   HIPCHK(ctx->kvals.ensure(sizeof(int) * num_k));
   HIPCHK(ctx->losses.ensure(sizeof(int) * num_loss));
   HIPCHK(ctx->state.ensure(sizeof(SelState)));
-  // precision form, forward: exact pruning seeded by the chosen candidate's
-  // lists of the previous step
+  HIPCHK(ctx->perm.ensure(sizeof(int) * d));
+  HIPCHK(ctx->key.ensure(sizeof(double) * d));
+  // precision form, forward: exact pruning with bounds from 128 seeds per query
   const bool pruned = comp && fwd;
-  const int K = snn::search_list_size(kmax);
   if (pruned) {
-    HIPCHK(ctx->NN.ensure(sizeof(int) * (size_t)n * pb.cpad * K));
-    HIPCHK(ctx->seeds.ensure(sizeof(int) * (size_t)n * K));
+    HIPCHK(ctx->seeds.ensure(sizeof(int) * (size_t)n * 128));
     HIPCHK(ctx->visited.ensure(sizeof(unsigned long long)));
     HIPCHK(hipMemsetAsync(ctx->visited.p, 0, sizeof(unsigned long long), ctx->stream));
   }
@@ -196,8 +195,12 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   float* D = ctx->D.as<float>();
   float* Dlo = comp ? ctx->Dlo.as<float>() : nullptr;
   HIPCHK(snn::launch_transpose(dX, ctx->XT.as<float>(), n, d, s));
+  // candidate slots: ascending feature index (plain), grouped by scale (pruned)
+  HIPCHK(snn::launch_feature_order(ctx->XT.as<float>(), n, d, ctx->key.as<double>(),
+                                   ctx->perm.as<int>(), pruned, s));
   HIPCHK(snn::launch_init_state(st, pb, ctx->cand.as<int>(), ctx->sel.as<int>(),
-                                ctx->ordering.as<int>(), ctx->errors.as<float>(), s));
+                                ctx->ordering.as<int>(), ctx->errors.as<float>(),
+                                ctx->perm.as<int>(), s));
   if (fwd) {
     HIPCHK(snn::launch_fill(D, (int64_t)nn, 0.0f, s));
     if (Dlo) HIPCHK(snn::launch_fill(Dlo, (int64_t)nn, 0.0f, s));
@@ -223,7 +226,6 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   sl.model_type = prm->model_type;
   sl.zero = 0;
   if (pruned) {
-    sl.NN = ctx->NN.as<int>();
     sl.seeds = ctx->seeds.as<int>();
     sl.visited = ctx->visited.as<unsigned long long>();
   }
@@ -236,6 +238,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     HIPCHK(snn::launch_compact(dX, ctx->cand.as<int>(), st, ctx->Xc.as<float>(), n, d, pb.cpad, s));
     HIPCHK(hipEventRecord(b, s));
     if (pruned && step > 0) {
+      HIPCHK(snn::launch_seed_scan(D, st, ctx->seeds.as<int>(), n, pb.npf, s));
       HIPCHK(snn::launch_search_pruned(sl, s));
     } else {
       sl.zero = pruned ? 1 : 0;  // first forward step: D is 0
@@ -247,9 +250,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     HIPCHK(snn::launch_select(st, pb, ctx->kvals.as<int>(), ctx->err.as<float>(),
                               ctx->cand.as<int>(), ctx->sel.as<int>(), ctx->ordering.as<int>(),
                               ctx->errors.as<float>(), ctx->val.as<float>(), ctx->bestk.as<int>(),
-                              cand_err, s));
-    if (pruned)
-      HIPCHK(snn::launch_seeds(ctx->NN.as<int>(), st, ctx->seeds.as<int>(), n, pb.cpad, kmax, s));
+                              cand_err, ctx->perm.as<int>(), s));
     HIPCHK(hipEventRecord(u, s));
     HIPCHK(snn::launch_update(D, Dlo, ctx->XT.as<float>(), n, st, -1, fwd ? 1 : -1, s));
     marks.push_back(a);
@@ -385,7 +386,8 @@ void snn_destroy(snn_ctx* c) {
   hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->X, &c->y, &c->XT, &c->D, &c->Dlo, &c->Xc, &c->P, &c->err, &c->val,
                     &c->bestk, &c->cand, &c->sel, &c->ordering, &c->errors, &c->cand_err,
-                    &c->kvals, &c->losses, &c->state, &c->scratch};
+                    &c->kvals, &c->losses, &c->state, &c->scratch, &c->NN, &c->seeds,
+                    &c->visited, &c->perm, &c->key};
   for (DevBuf* b : bufs) b->release();
   hipStreamDestroy(c->stream);
   delete c;

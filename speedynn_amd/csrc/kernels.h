@@ -36,8 +36,8 @@ hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* 
                           int d, int cpad, hipStream_t s);
 hipError_t launch_search_dense(const SearchLaunch& a, hipStream_t s);
 hipError_t launch_search_pruned(const SearchLaunch& a, hipStream_t s);
-hipError_t launch_seeds(const int* NN, const SelState* st, int* seeds, int64_t n, int cpad,
-                        int kmax, hipStream_t s);
+hipError_t launch_seed_scan(const float* D, const SelState* st, int* seeds, int64_t n,
+                            int64_t npf, hipStream_t s);
 hipError_t launch_loss(const float* P, const float* y, const int* losses, const SelState* st,
                        float* err, int64_t n, int cpad, int nk, int nl, int max_cand,
                        hipStream_t s);
@@ -45,8 +45,10 @@ hipError_t launch_loss_single(const float* y, const float* p, int64_t n, int cod
                               hipStream_t s);
 hipError_t launch_select(SelState* st, const Problem& pb, const int* kvals, const float* err,
                          int* cand, int* sel, int* ordering, float* errors, float* val,
-                         int* bestk, float* cand_err, hipStream_t s);
+                         int* bestk, float* cand_err, const int* perm, hipStream_t s);
+hipError_t launch_feature_order(const float* XT, int64_t n, int d, double* key, int* perm,
+                                bool by_scale, hipStream_t s);
 hipError_t launch_init_state(SelState* st, const Problem& pb, int* cand, int* sel, int* ordering,
-                             float* errors, hipStream_t s);
+                             float* errors, const int* perm, hipStream_t s);
 
 }  // namespace snn
