@@ -68,7 +68,7 @@ struct snn_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   DevBuf X, y, XT, D, Dlo, Xc, P, err, val, bestk, cand, sel, ordering, errors, cand_err, kvals,
-      losses, state, scratch, NN, seeds, visited, perm, key;
+      losses, state, scratch, seeds, visited, perm, key, M;
 };
 
 namespace {
@@ -137,6 +137,8 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   pb.npf = n / prm->k_fold;
   pb.d = d;
   pb.cpad = (d + 63) / 64 * 64;
+  if (n * (int64_t)pb.cpad >= ((int64_t)1 << 31))
+    return fail(SNN_ENOTSUP, "n * padded d must stay below 2^31 (32-bit gather offsets)");
   pb.nk = num_k;
   pb.nl = num_loss;
   pb.kmax = kmax;
@@ -168,7 +170,9 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   HIPCHK(ctx->perm.ensure(sizeof(int) * d));
   HIPCHK(ctx->key.ensure(sizeof(double) * d));
   // precision form, forward: exact pruning with bounds from 128 seeds per query
-  const bool pruned = comp && fwd;
+  // (backward also keeps M, the max single-feature term, for its lower bound)
+  const bool pruned = comp;
+  if (pruned && !fwd) HIPCHK(ctx->M.ensure(sizeof(float) * nn));
   if (pruned) {
     HIPCHK(ctx->seeds.ensure(sizeof(int) * (size_t)n * 128));
     HIPCHK(ctx->visited.ensure(sizeof(unsigned long long)));
@@ -205,7 +209,8 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     HIPCHK(snn::launch_fill(D, (int64_t)nn, 0.0f, s));
     if (Dlo) HIPCHK(snn::launch_fill(Dlo, (int64_t)nn, 0.0f, s));
   } else {
-    HIPCHK(snn::launch_gram_exact(ctx->XT.as<float>(), D, Dlo, n, d, s));
+    HIPCHK(snn::launch_gram_exact(ctx->XT.as<float>(), D, Dlo, pruned ? ctx->M.as<float>() : nullptr,
+                                  n, d, s));
   }
   HIPCHK(hipEventRecord(e1, s));
 
@@ -216,7 +221,6 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   sl.kvals = ctx->kvals.as<int>();
   sl.st = st;
   sl.P = ctx->P.as<float>();
-  sl.NN = nullptr;
   sl.n = n;
   sl.npf = pb.npf;
   sl.cpad = pb.cpad;
@@ -226,6 +230,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   sl.model_type = prm->model_type;
   sl.zero = 0;
   if (pruned) {
+    sl.M = fwd ? nullptr : ctx->M.as<float>();
     sl.seeds = ctx->seeds.as<int>();
     sl.visited = ctx->visited.as<unsigned long long>();
   }
@@ -237,7 +242,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     HIPCHK(hipEventRecord(a, s));
     HIPCHK(snn::launch_compact(dX, ctx->cand.as<int>(), st, ctx->Xc.as<float>(), n, d, pb.cpad, s));
     HIPCHK(hipEventRecord(b, s));
-    if (pruned && step > 0) {
+    if (pruned && (step > 0 || !fwd)) {
       HIPCHK(snn::launch_seed_scan(D, st, ctx->seeds.as<int>(), n, pb.npf, s));
       HIPCHK(snn::launch_search_pruned(sl, s));
     } else {
@@ -296,7 +301,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     int64_t ev = 0, pruned_full = 0;
     for (int t = 0; t < hs.step; ++t) {
       ev += (int64_t)(d - t) * n;
-      if (pruned && t > 0) pruned_full += (int64_t)(d - t) * n * (n - pb.npf);
+      if (pruned && (t > 0 || !fwd)) pruned_full += (int64_t)(d - t) * n * (n - pb.npf);
     }
     summary->evals = ev;
     if (pruned) {
@@ -386,8 +391,8 @@ void snn_destroy(snn_ctx* c) {
   hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->X, &c->y, &c->XT, &c->D, &c->Dlo, &c->Xc, &c->P, &c->err, &c->val,
                     &c->bestk, &c->cand, &c->sel, &c->ordering, &c->errors, &c->cand_err,
-                    &c->kvals, &c->losses, &c->state, &c->scratch, &c->NN, &c->seeds,
-                    &c->visited, &c->perm, &c->key};
+                    &c->kvals, &c->losses, &c->state, &c->scratch, &c->seeds,
+                    &c->visited, &c->perm, &c->key, &c->M};
   for (DevBuf* b : bufs) b->release();
   hipStreamDestroy(c->stream);
   delete c;
@@ -448,7 +453,7 @@ int snn_distance_rows(snn_ctx* ctx, const float* X, int64_t n, int d, int method
     HIPCHK(snn::launch_fill(D, (int64_t)nn, 0.0f, st));
     if (Dlo) HIPCHK(snn::launch_fill(Dlo, (int64_t)nn, 0.0f, st));
   } else {
-    HIPCHK(snn::launch_gram_exact(ctx->XT.as<float>(), D, Dlo, n, d, st));
+    HIPCHK(snn::launch_gram_exact(ctx->XT.as<float>(), D, Dlo, nullptr, n, d, st));
   }
   for (int t = 0; t < s; ++t)
     HIPCHK(snn::launch_update(D, Dlo, ctx->XT.as<float>(), n, nullptr, order[t],

@@ -11,8 +11,8 @@ struct SearchLaunch {
   const int* kvals;
   const SelState* st;
   float* P;
-  int* NN;        // optional [n][cpad][K] final lists (-1: unfilled slot)
-  const int* seeds;  // pruned search: [n][K] previous chosen lists
+  const int* seeds;  // pruned search: [n][128] bound seeds per query
+  const float* M;    // backward pruned search: [n][n] max single-feature term
   unsigned long long* visited;  // pruned search: (query, point) pairs examined
   int64_t n;
   int64_t npf;
@@ -28,7 +28,7 @@ int search_list_size(int kmax);
 
 hipError_t launch_fill(float* p, int64_t count, float v, hipStream_t s);
 hipError_t launch_transpose(const float* X, float* XT, int64_t n, int d, hipStream_t s);
-hipError_t launch_gram_exact(const float* XT, float* D, float* Dlo, int64_t n, int d,
+hipError_t launch_gram_exact(const float* XT, float* D, float* Dlo, float* M, int64_t n, int d,
                              hipStream_t s);
 hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const SelState* st,
                          int f, int mult, hipStream_t s);

@@ -72,21 +72,23 @@ __device__ __forceinline__ float comp_error(float a, float b, float t, float p, 
 // (TwoProduct/TwoSum error terms), so D + Dlo carries ~2x the precision.
 // 64x64 output tile per 256-thread block, 4x4 per thread, X^T chunks of 16
 // features staged in LDS.
-template <bool COMP>
+// With MAXM, M[i][j] = max over features of fl((x_ik - x_jk)^2) is written
+// too: the backward pruned search bounds d = fl(D - q_f) >= fl(D - M).
+template <bool COMP, bool MAXM>
 __global__ __launch_bounds__(256) void k_gram_exact(const float* __restrict__ XT,
                                                     float* __restrict__ D, float* __restrict__ Dlo,
-                                                    int64_t n, int d) {
+                                                    float* __restrict__ M, int64_t n, int d) {
   constexpr int KC = 16;
   __shared__ float As[KC][64];
   __shared__ float Bs[KC][64];
   const int64_t i0 = (int64_t)blockIdx.y * 64;
   const int64_t j0 = (int64_t)blockIdx.x * 64;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  float acc[4][4], lo[4][4];
+  float acc[4][4], lo[4][4], mx[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) { acc[a][b] = 0.0f; lo[a][b] = 0.0f; }
+    for (int b = 0; b < 4; ++b) { acc[a][b] = 0.0f; lo[a][b] = 0.0f; mx[a][b] = 0.0f; }
 
   for (int kc = d; kc > 0; kc -= KC) {
     const int kb = kc - KC > 0 ? kc - KC : 0;
@@ -117,6 +119,7 @@ __global__ __launch_bounds__(256) void k_gram_exact(const float* __restrict__ XT
           const float p = t * t;
           const float s = acc[a][b] + p;
           if (COMP) lo[a][b] += comp_error(a4[a], b4[b], t, p, acc[a][b], p, s, false);
+          if (MAXM) mx[a][b] = fmaxf(mx[a][b], p);
           acc[a][b] = s;
         }
     }
@@ -132,6 +135,7 @@ __global__ __launch_bounds__(256) void k_gram_exact(const float* __restrict__ XT
       if (j < n) {
         D[i * n + j] = acc[a][b];
         if (COMP) Dlo[i * n + j] = lo[a][b];
+        if (MAXM) M[i * n + j] = mx[a][b];
       }
     }
   }
@@ -489,11 +493,14 @@ hipError_t launch_transpose(const float* X, float* XT, int64_t n, int d, hipStre
   return hipGetLastError();
 }
 
-hipError_t launch_gram_exact(const float* XT, float* D, float* Dlo, int64_t n, int d,
+hipError_t launch_gram_exact(const float* XT, float* D, float* Dlo, float* M, int64_t n, int d,
                              hipStream_t s) {
   dim3 g((unsigned)((n + 63) / 64), (unsigned)((n + 63) / 64));
-  if (Dlo) hipLaunchKernelGGL(k_gram_exact<true>, g, dim3(256), 0, s, XT, D, Dlo, n, d);
-  else hipLaunchKernelGGL(k_gram_exact<false>, g, dim3(256), 0, s, XT, D, Dlo, n, d);
+  dim3 b(256);
+  if (Dlo && M) hipLaunchKernelGGL((k_gram_exact<true, true>), g, b, 0, s, XT, D, Dlo, M, n, d);
+  else if (Dlo) hipLaunchKernelGGL((k_gram_exact<true, false>), g, b, 0, s, XT, D, Dlo, M, n, d);
+  else if (M) hipLaunchKernelGGL((k_gram_exact<false, true>), g, b, 0, s, XT, D, Dlo, M, n, d);
+  else hipLaunchKernelGGL((k_gram_exact<false, false>), g, b, 0, s, XT, D, Dlo, M, n, d);
   return hipGetLastError();
 }
 

@@ -61,23 +61,15 @@ __device__ __forceinline__ float predict(const float* lab, int k, int model_type
 
 // Epilogue of one (query, candidate) lane: the list's labels go to the
 // wave's LDS column `lab`, then the prediction of every k value is written to
-// P[m][c][q].  The final list (index, or -1 for an unfilled slot) goes to NN
-// when requested: the next forward step seeds its pruning bound with the list
-// of the chosen candidate.
+// P[m][c][q].
 template <int K>
-__device__ __forceinline__ void finish_query(const float (&Ld)[K], const int (&Li)[K], float* lab,
+__device__ __forceinline__ void finish_query(const int (&Li)[K], float* lab,
                                              const float* __restrict__ y,
                                              const int* __restrict__ kvals, int nk,
-                                             int model_type, float* __restrict__ P,
-                                             int* __restrict__ NN, int64_t q, int c, int cpad,
-                                             int64_t n, bool live) {
+                                             int model_type, float* __restrict__ P, int64_t q,
+                                             int c, int cpad, int64_t n, bool live) {
 #pragma unroll
   for (int m = 0; m < K; ++m) lab[m * 64] = y[Li[m]];
-  if (NN && live) {
-    int* out = NN + (q * cpad + c) * K;
-#pragma unroll
-    for (int m = 0; m < K; ++m) out[m] = Ld[m] < kMaxF ? Li[m] : -1;
-  }
   for (int m = 0; m < nk; ++m) {
     const int k = kvals[m];
     const float pred = predict(lab, k, model_type);
@@ -98,7 +90,7 @@ template <int K, int QW, int WAVES, int MULT, bool ZERO>
 __global__ __launch_bounds__(64 * WAVES) void k_search_dense(
     const float* __restrict__ D, const float* __restrict__ Xc, const float* __restrict__ y,
     const int* __restrict__ kvals, const SelState* __restrict__ st, float* __restrict__ P,
-    int* __restrict__ NN, int64_t n, int64_t npf, int cpad, int nk, int model_type) {
+    int64_t n, int64_t npf, int cpad, int nk, int model_type) {
   constexpr int J = 64;
   constexpr int LDS_SCAN = 2 * J * 64;
   constexpr int LDS_EPI = WAVES * K * 64;
@@ -203,7 +195,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_search_dense(
   for (int qi = 0; qi < QW; ++qi) {
     const int64_t q = qbase + qi;
     if (q >= n) break;
-    finish_query<K>(Ld[qi], Li[qi], lab, y, kvals, nk, model_type, P, NN, q, c, cpad, n, live);
+    finish_query<K>(Li[qi], lab, y, kvals, nk, model_type, P, q, c, cpad, n, live);
   }
 }
 
@@ -257,45 +249,69 @@ __device__ __forceinline__ void kth_insert(float (&B)[K], float v) {
   B[0] = v < B[0] ? v : B[0];
 }
 
-// Pruned forward search with an exact bound (forward steps after the first).
+
+// Pruned search with an exact bound (precision form: forward steps after the
+// first, every backward step).
 //
-// Forward distances only grow: d_qjf = fl(D_qj + fl(t^2)) >= D_qj.  For a
-// candidate f, the K-th smallest d_qjf over any K distinct eligible points is
-// an upper bound tau_f of its final K-th distance; here it is the K-th
-// smallest over the 128 seeds of k_seed_scan, evaluated exactly.  A point
-// with D_qj > T = max over the wave's candidates of tau_f can enter no lane's
-// final top-K and is skipped before any per-candidate work; T tightens to the
-// lists' own K-th distance as they fill.  Points are still offered in
-// ascending index order, so the lists equal the dense search's bit for bit.
+// Bound.  For a candidate f, the K-th smallest d_qjf over any K distinct
+// eligible points is an upper bound tau_f of its final K-th distance; here it
+// is the K-th smallest over the 128 seeds of k_seed_scan, evaluated exactly.
+// Each point has a lower bound lb_qj of d_qjf valid for every candidate:
+//   forward  (MULT = +1): d = fl(D + fl(t^2)) >= D, so lb = D;
+//   backward (MULT = -1): M_qj >= fl(t^2) for every feature (max single-
+//     feature term, computed with the full matrix), so d = fl(D - fl(t^2)) >=
+//     fl(D - M) = lb by monotone rounding.
+// A point with lb > T_w = max over wave w's candidates of tau_f can enter no
+// lane's final top-K and is skipped; T_w tightens to the lists' own K-th
+// distance as they fill.  Points are still offered in ascending index order,
+// so every list equals the dense search's bit for bit.
 //
-// One block = one query; wave w = candidate slots [64w, 64w+64), so the D row
-// comes from HBM once per query and the other waves hit in cache.  Per 64
-// points: one coalesced load of D_q[j0:j0+64], a ballot of the points with
-// D_qj <= T outside the query's fold, then for each such point (ascending)
-// the candidates' values Xc[j][64w+lane] (256 B, coalesced) and the distance.
-// A fused multiply-add screen fma(t,t,D) is compared against the K-th
-// distance widened by 2^-20 (plus 1e-36 for the subnormal range);
+// Layout.  One block = one query; wave w = candidate slots [64w', 64w'+64)
+// with w' = blockIdx.y*W + w, so the D (and M) row comes from HBM once per
+// block and the other waves hit in cache.  Per 64 points a wave takes the
+// ballot of lb <= T_w outside the query's fold (the lower bounds of the next
+// four chunks are already in flight), then for each such point, ascending,
+// gathers the candidates' values Xc[j][64w'+lane] (256 B, coalesced) and
+// evaluates d.  No block barriers.  Forward tests use a screen fma(t,t,D) against the
+// K-th distance widened by 2^-20 (plus 1e-36 for the subnormal range):
 // |fl(D+fl(t*t)) - fma(t,t,D)| <= 3.01*2^-24*(D+t*t) for D, t*t >= 0, so a
-// point the screen rejects cannot pass the exact test.  Screen passes
-// recompute d in the reference's order before inserting.
-template <int K>
-__global__ __launch_bounds__(1024) void k_search_pruned(
-    const float* __restrict__ D, const float* __restrict__ Xc, const float* __restrict__ y,
-    const int* __restrict__ kvals, const SelState* __restrict__ st,
-    const int* __restrict__ seeds, float* __restrict__ P, unsigned long long* __restrict__ visited,
-    int64_t n, int64_t npf, int cpad, int nk, int model_type) {
-  extern __shared__ float lds[];  // epilogue labels: waves x K x 64
+// rejected point cannot pass the exact test; passes recompute d in the
+// reference's order.  Backward tests are exact (no screen: the fma error is
+// not bounded relative to D - t^2).
+template <int K, int MULT>
+__global__ __launch_bounds__(256) void k_search_pruned(
+    const float* __restrict__ D, const float* __restrict__ M, const float* __restrict__ Xc,
+    const float* __restrict__ y, const int* __restrict__ kvals, const SelState* __restrict__ st,
+    const int* __restrict__ seeds, float* __restrict__ P,
+    unsigned long long* __restrict__ visited, int64_t n, int64_t npf, int cpad, int nk,
+    int model_type, int cb_major) {
+  extern __shared__ float smem[];  // epilogue labels: waves x K x 64
   if (st->stopped) return;
   const int ncand = st->ncand;
+  const int W = blockDim.x >> 6;
   const int lane = threadIdx.x & 63;
   const int wave = uniform(threadIdx.x >> 6);
-  const int cb = blockIdx.y * (blockDim.x >> 6) + wave;
-  if (cb * 64 >= ncand) return;  // whole wave; this kernel has no block barriers
+  // Every wave is an independent (query, 64-candidate block) pair.  Blocks are
+  // dealt to the 8 XCDs round-robin; XCD x takes the x-th contiguous slice of
+  // the queries.  cb_major: all queries of a slice for candidate block 0,
+  // then block 1, ... -- concurrent waves gather from one 64-column slice of
+  // Xc (measured 1.6x faster on the n=65536 forward selection).  Otherwise the
+  // nblk blocks of one query are consecutive waves that read the query's D
+  // (and M) row once into L2.
+  const int nblk = cpad / 64;
+  const int64_t per_xcd = (n + 7) / 8;
+  const int64_t local = (int64_t)(blockIdx.x / 8) * W + wave;
+  const int64_t qi = cb_major ? local % per_xcd : local / nblk;
+  const int cb = (int)(cb_major ? local / per_xcd : local % nblk);
+  const int64_t pos = (int64_t)(blockIdx.x % 8) * per_xcd + qi;
+  if (pos >= n || cb >= nblk || qi >= per_xcd) return;  // whole wave; no block barriers
+  if (cb * 64 >= ncand) return;
   const int c = cb * 64 + lane;
   const bool live = c < ncand;
-  const int64_t q = blockIdx.x;
+  const int64_t q = pos;
 
   const float* __restrict__ Dq = D + q * n;
+  const float* __restrict__ Mq = MULT > 0 ? nullptr : M + q * n;
   const float* __restrict__ Xl = Xc + cb * 64 + lane;
   const float xq = Xl[q * cpad];
   const int64_t flo = (q / npf) * npf;
@@ -303,34 +319,37 @@ __global__ __launch_bounds__(1024) void k_search_pruned(
   const float kInf = __int_as_float(0x7f800000);
   constexpr float kScreen = 1.00000095367431640625f;  // 1 + 2^-20
   constexpr float kTiny = 1.0e-36f;
+  auto dist = [](float dq, float p) { return MULT > 0 ? dq + p : dq - p; };
 
   // bound: K-th smallest exact distance over the seeds
   float B[K];
 #pragma unroll
   for (int m = 0; m < K; ++m) B[m] = kInf;
-  const int* __restrict__ sq = seeds + q * 128;
-  for (int s0 = 0; s0 < 128; s0 += 8) {
-    int js[8];
-    float xs[8], dqs[8];
+  {
+    const int* __restrict__ sq = seeds + q * 128;
+    for (int s0 = 0; s0 < 128; s0 += 8) {
+      int js[8];
+      float xs[8], dqs[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      js[u] = sq[s0 + u];
-      const int jj = js[u] < 0 ? 0 : js[u];
-      xs[u] = Xl[(int64_t)jj * cpad];
-      dqs[u] = Dq[jj];
-    }
+      for (int u = 0; u < 8; ++u) {
+        js[u] = sq[s0 + u];
+        const int jj = js[u] < 0 ? 0 : js[u];
+        xs[u] = Xl[(int64_t)jj * cpad];
+        dqs[u] = Dq[jj];
+      }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (js[u] >= 0) {
-        const float t = xq - xs[u];
-        const float p = t * t;
-        const float dd = dqs[u] + p;
-        if (dd < B[K - 1]) kth_insert<K>(B, dd);
+      for (int u = 0; u < 8; ++u) {
+        if (js[u] >= 0) {
+          const float t = xq - xs[u];
+          const float p = t * t;
+          const float dd = dist(dqs[u], p);
+          if (dd < B[K - 1]) kth_insert<K>(B, dd);
+        }
       }
     }
   }
   float thr = live ? B[K - 1] : -kInf;
-  float T = wave_max(thr);
+  float Tw = wave_max(thr);
 
   float Ld[K];
   int Li[K];
@@ -340,13 +359,31 @@ __global__ __launch_bounds__(1024) void k_search_pruned(
     Li[m] = 0;
   }
   // screen limit: live lanes pass every point until their list is full
-  // (+inf); padding lanes (c >= ncand) never pass (-inf)
+  // (+inf); padding lanes never pass (-inf)
   float lim = live ? kInf : -kInf;
   unsigned long long seen = 0;
+
+  // D (and M) of 64 points per chunk; a window of 4 chunks is in flight
+  auto ldD = [&](int64_t j) { return j < n ? Dq[j] : kInf; };
+  auto ldM = [&](int64_t j) { return (MULT < 0 && j < n) ? Mq[j] : 0.0f; };
+  float d0 = ldD(lane), d1 = ldD(64 + lane), d2 = ldD(128 + lane), d3 = ldD(192 + lane);
+  float m0 = ldM(lane), m1 = ldM(64 + lane), m2 = ldM(128 + lane), m3 = ldM(192 + lane);
+  const uint32_t ucpad = (uint32_t)cpad;  // n * cpad < 2^31 (checked on the host)
   for (int64_t j0 = 0; j0 < n; j0 += 64) {
     const int64_t jl = j0 + lane;
-    const float Dv = jl < n ? Dq[jl] : kInf;
-    const bool ok = jl < n && (jl < flo || jl >= fhi) && Dv <= T;
+    const float dcur = d0;
+    const float lb = MULT > 0 ? d0 : d0 - m0;
+    d0 = d1;
+    d1 = d2;
+    d2 = d3;
+    d3 = ldD(jl + 256);
+    if (MULT < 0) {
+      m0 = m1;
+      m1 = m2;
+      m2 = m3;
+      m3 = ldM(jl + 256);
+    }
+    const bool ok = jl < n && (jl < flo || jl >= fhi) && lb <= Tw;
     unsigned long long mask = __ballot(ok);
     if (!mask) continue;
     seen += __popcll(mask);
@@ -354,30 +391,39 @@ __global__ __launch_bounds__(1024) void k_search_pruned(
     while (mask) {
       constexpr int BATCH = 8;
       int js[BATCH];
-      float dqs[BATCH], xs[BATCH];
+      float dqs[BATCH], ts[BATCH];
+      bool pass[BATCH];
       int cnt = 0;
 #pragma unroll
       for (int u = 0; u < BATCH; ++u) {
         if (mask) {
           const int b = __ffsll((long long)mask) - 1;
           mask &= mask - 1;
-          js[u] = (int)(j0 + b);
-          dqs[u] = lane_value(Dv, b);
-          xs[u] = Xl[(int64_t)js[u] * cpad];
+          js[u] = (int)j0 + b;
+          dqs[u] = lane_value(dcur, b);
+          ts[u] = Xl[(uint32_t)js[u] * ucpad];
           cnt = u + 1;
         }
       }
+      // screen the batch; the exact path runs only if some lane may pass
+      bool any = false;
 #pragma unroll
       for (int u = 0; u < BATCH; ++u) {
-        if (u < cnt) {
-          const float t = xq - xs[u];
-          const float screen = fmaf(t, t, dqs[u]);
-          if (screen < lim) {
-            const float p = t * t;
-            const float dd = dqs[u] + p;
+        ts[u] = xq - ts[u];
+        const float scr = MULT > 0 ? fmaf(ts[u], ts[u], dqs[u]) : dist(dqs[u], ts[u] * ts[u]);
+        pass[u] = u < cnt && scr < lim;
+        any |= pass[u];
+      }
+      if (__ballot(any)) {
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) {
+          if (pass[u]) {
+            const float p = ts[u] * ts[u];
+            const float dd = dist(dqs[u], p);
             if (dd < Ld[K - 1]) {
               topk_insert<K>(Ld, Li, dd, js[u]);
-              lim = Ld[K - 1] < kMaxF ? fmaf(Ld[K - 1], kScreen, kTiny) : kInf;
+              lim = MULT > 0 ? (Ld[K - 1] < kMaxF ? fmaf(Ld[K - 1], kScreen, kTiny) : kInf)
+                             : Ld[K - 1];
               changed = true;
             }
           }
@@ -386,15 +432,15 @@ __global__ __launch_bounds__(1024) void k_search_pruned(
     }
     if (__ballot(changed)) {
       thr = live ? fminf(thr, Ld[K - 1]) : -kInf;
-      T = wave_max(thr);
+      Tw = wave_max(thr);
     }
   }
   if (visited && lane == 0) {
     const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
     atomicAdd(visited, seen * (unsigned long long)nlive);  // point evaluations performed
   }
-  float* lab = lds + wave * (K * 64) + lane;
-  finish_query<K>(Ld, Li, lab, y, kvals, nk, model_type, P, nullptr, q, c, cpad, n, live);
+  float* lab = smem + wave * (K * 64) + lane;
+  finish_query<K>(Li, lab, y, kvals, nk, model_type, P, q, c, cpad, n, live);
 }
 
 // ============================================================ launchers
@@ -406,27 +452,32 @@ static hipError_t launch_dense_k(const SearchLaunch& a, hipStream_t s) {
   dim3 b(64 * WAVES);
   if (a.mult < 0)
     hipLaunchKernelGGL((k_search_dense<K, QW, WAVES, -1, false>), g, b, 0, s, a.D, a.Xc, a.y,
-                       a.kvals, a.st, a.P, a.NN, a.n, a.npf, a.cpad, a.nk, a.model_type);
+                       a.kvals, a.st, a.P, a.n, a.npf, a.cpad, a.nk, a.model_type);
   else if (a.zero)
     hipLaunchKernelGGL((k_search_dense<K, QW, WAVES, 1, true>), g, b, 0, s, a.D, a.Xc, a.y,
-                       a.kvals, a.st, a.P, a.NN, a.n, a.npf, a.cpad, a.nk, a.model_type);
+                       a.kvals, a.st, a.P, a.n, a.npf, a.cpad, a.nk, a.model_type);
   else
     hipLaunchKernelGGL((k_search_dense<K, QW, WAVES, 1, false>), g, b, 0, s, a.D, a.Xc, a.y,
-                       a.kvals, a.st, a.P, a.NN, a.n, a.npf, a.cpad, a.nk, a.model_type);
+                       a.kvals, a.st, a.P, a.n, a.npf, a.cpad, a.nk, a.model_type);
   return hipGetLastError();
 }
 
 template <int K>
 static hipError_t launch_pruned_k(const SearchLaunch& a, hipStream_t s) {
-  // waves of a block share the query; epilogue LDS (waves*K*256 B) stays <= 64 KB
-  const int nblk = a.cpad / 64;
-  int waves = nblk < 16 ? nblk : 16;
-  if (waves > 256 / K) waves = 256 / K;
-  const unsigned gy = (unsigned)((nblk + waves - 1) / waves);
+  // 4 independent waves per block; each XCD slice holds per_xcd*nblk waves
+  const int waves = 4;
+  const int64_t per_xcd = (a.n + 7) / 8;
+  const int64_t slice_blocks = (per_xcd * (a.cpad / 64) + waves - 1) / waves;
   const size_t lds = sizeof(float) * (size_t)waves * K * 64;
-  hipLaunchKernelGGL((k_search_pruned<K>), dim3((unsigned)a.n, gy), dim3(64 * waves), lds, s, a.D,
-                     a.Xc, a.y, a.kvals, a.st, a.seeds, a.P, a.visited, a.n, a.npf, a.cpad, a.nk,
-                     a.model_type);
+  const dim3 g((unsigned)(8 * slice_blocks)), b(64 * waves);
+  // forward: gather locality (candidate-block major); backward: the D and M
+  // rows dominate (query major)
+  if (a.mult > 0)
+    hipLaunchKernelGGL((k_search_pruned<K, 1>), g, b, lds, s, a.D, a.M, a.Xc, a.y, a.kvals, a.st,
+                       a.seeds, a.P, a.visited, a.n, a.npf, a.cpad, a.nk, a.model_type, 1);
+  else
+    hipLaunchKernelGGL((k_search_pruned<K, -1>), g, b, lds, s, a.D, a.M, a.Xc, a.y, a.kvals, a.st,
+                       a.seeds, a.P, a.visited, a.n, a.npf, a.cpad, a.nk, a.model_type, 0);
   return hipGetLastError();
 }
 
