@@ -135,6 +135,39 @@ int snn_select_device(snn_ctx *ctx, const float *X_device, int64_t n, int d,
                       int *selected_dimensions, int *selected_ordering, float *selected_errors,
                       float *candidate_errors, snn_summary *summary);
 
+/* One selection problem of a batch: host inputs and host outputs. */
+typedef struct {
+    const float *X;              /* n x d row-major */
+    const float *y;              /* n labels */
+    int64_t n;
+    int d;
+    const int *k_vals;
+    int num_k;                   /* largest k value <= 24 in a batch */
+    const int *loss_functions;
+    int num_loss;
+    snn_params params;           /* form and use_graph must agree across the batch */
+    int *selected_dimensions;    /* outputs as in snn_select; each may be NULL */
+    int *selected_ordering;
+    float *selected_errors;
+    int status;                  /* out: SNN_OK or this problem's error code */
+    int num_final;               /* out */
+    int optimal_k;               /* out */
+    int steps;                   /* out */
+} snn_problem;
+
+/* Many independent selections (uneven n, d and k) advanced together: every
+ * step runs the candidate search of all problems in one persistent launch
+ * (dense for a forward selection's first step, pruned afterwards in the
+ * precision form), with the per-problem gather, losses, arg-min and update
+ * around it; with params.use_graph the step is captured once in a hipGraph
+ * and replayed.  Problems whose arguments are invalid get their status set and
+ * are left out; the call returns SNN_EINVAL if any was, after running the
+ * others.  No counterpart in the reference, which selects one problem per call
+ * (speedynn/lib/nearest_neighbors_feature_selection.c:49-65).  summary
+ * (optional) holds the batch totals. */
+int snn_select_batch(snn_ctx *ctx, int num_problems, snn_problem *problems,
+                     snn_summary *summary);
+
 /* Rows of the current distance matrix after applying `s` steps of `order`
  * from the method's start state (forward: D = 0 then + each feature;
  * backward: full matrix then - each feature).  rows_out is [nrows][n];

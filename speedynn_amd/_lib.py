@@ -22,7 +22,7 @@ EXPORTS = [
     "train_backward_selection", "train_forward_selection", "get_selected_dimensions",
     "get_selected_ordering", "get_selected_errors", "get_final_optimal_model_k", "get_loss_all",
     "snn_create", "snn_destroy", "snn_last_error", "snn_select", "snn_select_device",
-    "snn_distance_rows", "snn_loss", "snn_set_default_form",
+    "snn_distance_rows", "snn_loss", "snn_set_default_form", "snn_select_batch",
 ]
 
 
@@ -38,6 +38,15 @@ class SnnSummary(C.Structure):
                 ("ms_search", C.c_double), ("ms_update", C.c_double), ("ms_tally", C.c_double),
                 ("evals", C.c_int64),
                 ("pruned_point_evals", C.c_int64)]
+
+
+class SnnProblem(C.Structure):
+    _fields_ = [("X", C.c_void_p), ("y", C.c_void_p), ("n", C.c_int64), ("d", C.c_int),
+                ("k_vals", C.c_void_p), ("num_k", C.c_int), ("loss_functions", C.c_void_p),
+                ("num_loss", C.c_int), ("params", SnnParams),
+                ("selected_dimensions", C.c_void_p), ("selected_ordering", C.c_void_p),
+                ("selected_errors", C.c_void_p), ("status", C.c_int), ("num_final", C.c_int),
+                ("optimal_k", C.c_int), ("steps", C.c_int)]
 
 
 class EngineError(RuntimeError):
@@ -80,6 +89,8 @@ def lib() -> C.CDLL:
                                C.POINTER(C.c_float)]
         L.snn_set_default_form.argtypes = [C.c_int]
         L.snn_set_default_form.restype = None
+        L.snn_select_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(SnnProblem),
+                                       C.POINTER(SnnSummary)]
         i = C.c_int
         for name in ("train_backward_selection", "train_forward_selection"):
             fn = getattr(L, name)
@@ -164,6 +175,55 @@ class Engine:
         if cand is not None:
             out["candidate_errors"] = cand.reshape(d, d)
         return out
+
+    def select_batch(self, problems, *, form: int = 1, use_graph: bool = True,
+                     raise_on_error: bool = True):
+        """Run many selections together (snn_select_batch).
+
+        problems: list of dicts with X, y, k_vals, loss_codes, method, model_type,
+        k_fold, num_features and optionally auto_feature.  Returns (results,
+        summary): one dict per problem (status, num_final, optimal_k, steps,
+        selected_dimensions, selected_ordering, selected_errors) and the batch
+        totals."""
+        keep = []  # arrays referenced by the C structs stay alive for the call
+        arr = (SnnProblem * len(problems))()
+        outs = []
+        for i, pr in enumerate(problems):
+            X = np.ascontiguousarray(pr["X"], dtype=np.float32)
+            y = np.ascontiguousarray(pr["y"], dtype=np.float32).reshape(-1)
+            kv = np.ascontiguousarray(pr["k_vals"], dtype=np.int32)
+            lc = np.ascontiguousarray(pr["loss_codes"], dtype=np.int32)
+            n, d = X.shape
+            sel = np.zeros(d, np.int32)
+            order = np.zeros(d, np.int32)
+            errs = np.zeros(len(lc) * d, np.float32)
+            keep += [X, y, kv, lc]
+            outs.append((sel, order, errs))
+            q = arr[i]
+            q.X, q.y, q.n, q.d = X.ctypes.data, y.ctypes.data, n, d
+            q.k_vals, q.num_k = kv.ctypes.data, len(kv)
+            q.loss_functions, q.num_loss = lc.ctypes.data, len(lc)
+            q.params = SnnParams(method=int(pr["method"]), model_type=int(pr["model_type"]),
+                                 k_fold=int(pr["k_fold"]), num_features=int(pr["num_features"]),
+                                 auto_feature=int(bool(pr.get("auto_feature", False))),
+                                 form=int(form), use_graph=int(bool(use_graph)), reserved=0)
+            q.selected_dimensions = sel.ctypes.data
+            q.selected_ordering = order.ctypes.data
+            q.selected_errors = errs.ctypes.data
+        sm = SnnSummary()
+        rc = lib().snn_select_batch(self._ctx, len(problems), arr, C.byref(sm))
+        if raise_on_error:
+            check(rc)
+        results = []
+        for i, (sel, order, errs) in enumerate(outs):
+            q = arr[i]
+            results.append(dict(status=q.status, num_final=q.num_final, optimal_k=q.optimal_k,
+                                steps=q.steps, selected_dimensions=sel,
+                                selected_ordering=order, selected_errors=errs))
+        summary = dict(rc=rc, steps=sm.steps, ms_total=sm.ms_total, ms_distance=sm.ms_distance,
+                       ms_search=sm.ms_search, evals=sm.evals,
+                       pruned_point_evals=sm.pruned_point_evals)
+        return results, summary
 
     def distance_rows(self, X, method: int, order, rows, form: int = 0):
         X = np.ascontiguousarray(X, dtype=np.float32)

@@ -1,0 +1,439 @@
+// Batched selection driver: many independent selection problems advance one
+// step at a time together.  Per step, each problem's candidate gather, and
+// after the search its losses, arg-min and distance update, are small kernels
+// on the context's stream; the candidate search of all problems of one
+// direction is ONE persistent launch over a shared work queue (search.hip).
+// The whole step can be captured in a hipGraph and replayed: every kernel
+// reads the device-resident state of its problem, so a step's launches do not
+// depend on the host.  Graphs are kept while the batch keeps its shapes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/speedynn.h"
+#include "common.h"
+#include "engine.h"
+#include "kernels.h"
+
+using snn::DevBuf;
+using snn::fail;
+using snn::Problem;
+using snn::SelState;
+
+namespace {
+
+// Byte offsets of one problem's buffers in the batch arena.
+struct Slot {
+  int index;  // position in the caller's array
+  Problem pb;
+  int64_t work;
+  size_t X, y, XT, D, Dlo, M, Xc, P, err, val, bestk, cand, sel, ordering, errors, kvals, losses,
+      state, perm, key, seeds, med;
+  bool has_med;  // median losses: sort space for NaN-holding error vectors
+};
+
+size_t take(size_t& off, size_t bytes) {
+  const size_t o = off;
+  off += (bytes + 255) / 256 * 256;
+  return o;
+}
+
+template <class T>
+T* at(const DevBuf& b, size_t off) {
+  return reinterpret_cast<T*>(static_cast<char*>(b.p) + off);
+}
+
+// Problems of one direction: a contiguous range of the device problem array
+// (largest work first, so the queue hands out the long items early).
+struct Group {
+  int first = 0, count = 0;
+  int mult = 1;
+  size_t dense_start = 0, pruned_start = 0;  // prefix arrays in meta
+};
+
+struct Timing {
+  std::vector<hipEvent_t> ev;
+  hipEvent_t make() {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ev.push_back(e);
+    return e;
+  }
+  ~Timing() {
+    for (auto e : ev) hipEventDestroy(e);
+  }
+};
+
+double elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (a && b && hipEventElapsedTime(&ms, a, b) == hipSuccess) return ms;
+  return 0.0;
+}
+
+struct Plan {
+  snn_ctx* ctx;
+  std::vector<Slot> slots;  // device order: groups contiguous
+  std::vector<Group> groups;
+  bool pruned;
+  size_t probs_off, counter_off, visited_off;
+  int grid_dense, grid_pruned;
+
+  const DevBuf& A() const { return ctx->batch.arena; }
+  const DevBuf& Mt() const { return ctx->batch.meta; }
+
+  // Launches of one selection step (step > 0 differs only for a forward
+  // selection in the precision form: its first step runs the dense search
+  // with D = 0, later ones the pruned search).
+  int issue_step(bool first_step, hipStream_t s, hipEvent_t ev_search0,
+                 hipEvent_t ev_search1) const {
+    for (const Slot& sl : slots) {
+      const Problem& pb = sl.pb;
+      HIPCHK(snn::launch_compact(at<float>(A(), sl.X), at<int>(A(), sl.cand),
+                                 at<SelState>(A(), sl.state), at<float>(A(), sl.Xc), pb.n, pb.d,
+                                 pb.cpad, s));
+    }
+    if (ev_search0) HIPCHK(hipEventRecord(ev_search0, s));
+    auto* probs = at<snn::SearchProblem>(Mt(), probs_off);
+    auto* counter = at<unsigned long long>(Mt(), counter_off);
+    for (const Group& g : groups) {
+      if (!g.count) continue;
+      const bool use_pruned = pruned && !(g.mult > 0 && first_step);
+      if (use_pruned) {
+        for (int i = g.first; i < g.first + g.count; ++i) {
+          const Slot& sl = slots[i];
+          HIPCHK(snn::launch_seed_scan(at<float>(A(), sl.D), at<SelState>(A(), sl.state),
+                                       at<int>(A(), sl.seeds), sl.pb.n, sl.pb.npf, s));
+        }
+        HIPCHK(snn::launch_batch_search(probs + g.first, at<int64_t>(Mt(), g.pruned_start),
+                                        g.count, counter, 2, g.mult, grid_pruned, s));
+      } else {
+        const int mode = (pruned && g.mult > 0) ? 1 : 0;
+        HIPCHK(snn::launch_batch_search(probs + g.first, at<int64_t>(Mt(), g.dense_start),
+                                        g.count, counter, mode, g.mult, grid_dense, s));
+      }
+    }
+    if (ev_search1) HIPCHK(hipEventRecord(ev_search1, s));
+    for (const Slot& sl : slots) {
+      const Problem& pb = sl.pb;
+      SelState* st = at<SelState>(A(), sl.state);
+      HIPCHK(snn::launch_loss(at<float>(A(), sl.P), at<float>(A(), sl.y), at<int>(A(), sl.losses),
+                              st, at<float>(A(), sl.err), pb.n, pb.cpad, pb.nk, pb.nl, pb.d,
+                              sl.has_med ? at<float>(A(), sl.med) : nullptr, s));
+      HIPCHK(snn::launch_select(st, pb, at<int>(A(), sl.kvals), at<float>(A(), sl.err),
+                                at<int>(A(), sl.cand), at<int>(A(), sl.sel),
+                                at<int>(A(), sl.ordering), at<float>(A(), sl.errors),
+                                at<float>(A(), sl.val), at<int>(A(), sl.bestk), nullptr,
+                                at<int>(A(), sl.perm), s));
+      HIPCHK(snn::launch_update(at<float>(A(), sl.D), pruned ? at<float>(A(), sl.Dlo) : nullptr,
+                                at<float>(A(), sl.XT), pb.n, st, -1, pb.method == 1 ? 1 : -1, s));
+    }
+    return SNN_OK;
+  }
+
+  int capture(bool first_step, hipStream_t s, hipGraphExec_t* out) const {
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = issue_step(first_step, s, nullptr, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    HIPCHK(e);
+    const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    HIPCHK(ei);
+    return SNN_OK;
+  }
+};
+
+}  // namespace
+
+extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* probs,
+                                snn_summary* summary) {
+  if (!ctx) return fail(SNN_EINVAL, "null context");
+  if (num_problems < 1 || !probs) return fail(SNN_EINVAL, "no problems");
+  if (summary) std::memset(summary, 0, sizeof(*summary));
+
+  // ---- validate; invalid problems are reported and left out
+  std::vector<Slot> valid;
+  int form = -1, use_graph = 0;
+  std::string first_err;
+  for (int p = 0; p < num_problems; ++p) {
+    snn_problem& q = probs[p];
+    q.status = SNN_OK;
+    q.num_final = q.optimal_k = q.steps = 0;
+    Problem pb{};
+    int rc = (q.X && q.y) ? snn::validate_problem(q.n, q.d, q.k_vals, q.num_k, q.loss_functions,
+                                                  q.num_loss, &q.params, &pb)
+                          : fail(SNN_EINVAL, "null input");
+    if (!rc && snn::search_list_size(pb.kmax) > snn::batch_max_list_size())
+      rc = fail(SNN_ENOTSUP, "k values above " + std::to_string(snn::batch_max_list_size()) +
+                                 " are not supported in a batch");
+    if (!rc && form >= 0 &&
+        (q.params.form != form || (q.params.use_graph != 0) != (use_graph != 0)))
+      rc = fail(SNN_EINVAL, "form and use_graph must agree across the batch");
+    if (rc) {
+      q.status = rc;
+      if (first_err.empty()) first_err = "problem " + std::to_string(p) + ": " + snn_last_error();
+      continue;
+    }
+    if (form < 0) {
+      form = q.params.form;
+      use_graph = q.params.use_graph != 0;
+    }
+    Slot s{};
+    s.index = p;
+    s.pb = pb;
+    s.work = pb.n * pb.n * (int64_t)pb.cpad;
+    valid.push_back(s);
+  }
+  if (valid.empty()) return fail(SNN_EINVAL, first_err);
+
+  Plan plan{};
+  plan.ctx = ctx;
+  plan.pruned = form == SNN_FORM_PRECISION;
+  const bool pruned = plan.pruned;
+
+  // ---- device order: forward group, then backward group, each by work descending
+  for (int dir = 1; dir >= 0; --dir) {
+    Group g;
+    g.first = (int)plan.slots.size();
+    g.mult = dir == 1 ? 1 : -1;
+    std::vector<Slot> part;
+    for (const Slot& s : valid)
+      if (s.pb.method == dir) part.push_back(s);
+    std::stable_sort(part.begin(), part.end(),
+                     [](const Slot& a, const Slot& b) { return a.work > b.work; });
+    for (const Slot& s : part) plan.slots.push_back(s);
+    g.count = (int)part.size();
+    plan.groups.push_back(g);
+  }
+
+  // ---- arena layout
+  size_t off = 0;
+  for (Slot& s : plan.slots) {
+    const Problem& pb = s.pb;
+    const size_t n = (size_t)pb.n, d = (size_t)pb.d, cpad = (size_t)pb.cpad;
+    s.X = take(off, sizeof(float) * n * d);
+    s.y = take(off, sizeof(float) * n);
+    s.XT = take(off, sizeof(float) * n * d);
+    s.D = take(off, sizeof(float) * n * n);
+    s.Dlo = pruned ? take(off, sizeof(float) * n * n) : 0;
+    s.M = (pruned && pb.method == 0) ? take(off, sizeof(float) * n * n) : 0;
+    s.Xc = take(off, sizeof(float) * n * cpad);
+    s.P = take(off, sizeof(float) * pb.nk * cpad * n);
+    s.err = take(off, sizeof(float) * pb.nl * pb.nk * cpad);
+    s.val = take(off, sizeof(float) * pb.nl * cpad);
+    s.bestk = take(off, sizeof(int) * cpad);
+    s.cand = take(off, sizeof(int) * d);
+    s.sel = take(off, sizeof(int) * d);
+    s.ordering = take(off, sizeof(int) * d);
+    s.errors = take(off, sizeof(float) * pb.nl * d);
+    s.kvals = take(off, sizeof(int) * pb.nk);
+    s.losses = take(off, sizeof(int) * pb.nl);
+    s.state = take(off, sizeof(SelState));
+    s.perm = take(off, sizeof(int) * d);
+    s.key = take(off, sizeof(double) * d);
+    s.seeds = pruned ? take(off, sizeof(int) * n * 128) : 0;
+    const snn_problem& q = probs[s.index];
+    s.has_med = snn::needs_median_scratch(q.loss_functions, q.num_loss);
+    s.med = s.has_med ? take(off, sizeof(float) * 2 * n * cpad * pb.nk) : 0;
+  }
+  const size_t arena_bytes = off;
+  const int B = (int)plan.slots.size();
+  size_t moff = 0;
+  plan.probs_off = take(moff, sizeof(snn::SearchProblem) * B);
+  for (Group& g : plan.groups) {
+    g.dense_start = take(moff, sizeof(int64_t) * (g.count + 1));
+    g.pruned_start = take(moff, sizeof(int64_t) * (g.count + 1));
+  }
+  plan.counter_off = take(moff, sizeof(unsigned long long));
+  plan.visited_off = take(moff, sizeof(unsigned long long));
+
+  HIPCHK(hipSetDevice(ctx->device));
+  snn::BatchCache& cache = ctx->batch;
+  const void* old_arena = cache.arena.p;
+  const void* old_meta = cache.meta.p;
+  HIPCHK(cache.arena.ensure(arena_bytes));
+  HIPCHK(cache.meta.ensure(moff));
+  hipStream_t s = ctx->stream;
+
+  // ---- host copies of the search descriptors and work queues
+  std::vector<snn::SearchProblem> hp(B);
+  std::vector<char> hmeta(moff, 0);
+  for (int i = 0; i < B; ++i) {
+    const Slot& sl = plan.slots[i];
+    const Problem& pb = sl.pb;
+    snn::SearchProblem& a = hp[i];
+    a.D = at<float>(cache.arena, sl.D);
+    a.M = (pruned && pb.method == 0) ? at<float>(cache.arena, sl.M) : nullptr;
+    a.Xc = at<float>(cache.arena, sl.Xc);
+    a.y = at<float>(cache.arena, sl.y);
+    a.kvals = at<int>(cache.arena, sl.kvals);
+    a.st = at<SelState>(cache.arena, sl.state);
+    a.seeds = pruned ? at<int>(cache.arena, sl.seeds) : nullptr;
+    a.P = at<float>(cache.arena, sl.P);
+    a.visited = pruned ? at<unsigned long long>(cache.meta, plan.visited_off) : nullptr;
+    a.n = pb.n;
+    a.npf = pb.npf;
+    a.cpad = pb.cpad;
+    a.nk = pb.nk;
+    a.kmax = pb.kmax;
+    a.mult = pb.method == 1 ? 1 : -1;
+    a.model_type = pb.model_type;
+    a.zero = 0;
+  }
+  std::memcpy(hmeta.data() + plan.probs_off, hp.data(), sizeof(snn::SearchProblem) * B);
+  for (const Group& g : plan.groups) {
+    int64_t* ds = reinterpret_cast<int64_t*>(hmeta.data() + g.dense_start);
+    int64_t* ps = reinterpret_cast<int64_t*>(hmeta.data() + g.pruned_start);
+    ds[0] = ps[0] = 0;
+    for (int i = 0; i < g.count; ++i) {
+      ds[i + 1] = ds[i] + snn::batch_dense_items(hp[g.first + i]);
+      ps[i + 1] = ps[i] + snn::batch_pruned_items(hp[g.first + i]);
+    }
+  }
+  HIPCHK(hipMemcpyAsync(cache.meta.p, hmeta.data(), moff, hipMemcpyHostToDevice, s));
+  plan.grid_dense = snn::batch_grid(0);
+  plan.grid_pruned = snn::batch_grid(2);
+
+  // ---- inputs and per-problem start state
+  Timing tm;
+  hipEvent_t e0 = tm.make(), e1 = tm.make(), e2 = tm.make();
+  for (const Slot& sl : plan.slots) {
+    const snn_problem& q = probs[sl.index];
+    const Problem& pb = sl.pb;
+    HIPCHK(hipMemcpyAsync(at<float>(cache.arena, sl.X), q.X, sizeof(float) * pb.n * pb.d,
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(at<float>(cache.arena, sl.y), q.y, sizeof(float) * pb.n,
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(at<int>(cache.arena, sl.kvals), q.k_vals, sizeof(int) * pb.nk,
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(at<int>(cache.arena, sl.losses), q.loss_functions, sizeof(int) * pb.nl,
+                          hipMemcpyHostToDevice, s));
+  }
+  HIPCHK(hipEventRecord(e0, s));
+  if (pruned) HIPCHK(snn::launch_fill_u64(at<unsigned long long>(cache.meta, plan.visited_off), 1, s));
+  for (const Slot& sl : plan.slots) {
+    const Problem& pb = sl.pb;
+    const int64_t nn = pb.n * pb.n;
+    float* XT = at<float>(cache.arena, sl.XT);
+    float* D = at<float>(cache.arena, sl.D);
+    float* Dlo = pruned ? at<float>(cache.arena, sl.Dlo) : nullptr;
+    HIPCHK(snn::launch_transpose(at<float>(cache.arena, sl.X), XT, pb.n, pb.d, s));
+    HIPCHK(snn::launch_feature_order(XT, pb.n, pb.d, at<double>(cache.arena, sl.key),
+                                     at<int>(cache.arena, sl.perm), pruned, s));
+    HIPCHK(snn::launch_init_state(at<SelState>(cache.arena, sl.state), pb,
+                                  at<int>(cache.arena, sl.cand), at<int>(cache.arena, sl.sel),
+                                  at<int>(cache.arena, sl.ordering),
+                                  at<float>(cache.arena, sl.errors), at<int>(cache.arena, sl.perm),
+                                  s));
+    if (pb.method == 1) {
+      HIPCHK(snn::launch_fill(D, nn, 0.0f, s));
+      if (Dlo) HIPCHK(snn::launch_fill(Dlo, nn, 0.0f, s));
+    } else {
+      HIPCHK(snn::launch_gram_exact(XT, D, Dlo, pruned ? at<float>(cache.arena, sl.M) : nullptr,
+                                    pb.n, pb.d, s));
+    }
+  }
+  HIPCHK(hipEventRecord(e1, s));
+
+  // ---- steps
+  int max_steps = 0;
+  for (const Slot& sl : plan.slots) max_steps = std::max(max_steps, sl.pb.max_steps);
+  double ms_search = 0.0;
+  if (use_graph) {
+    std::vector<size_t> key{(size_t)cache.arena.p, (size_t)cache.meta.p, (size_t)form,
+                            (size_t)B};
+    for (const Slot& sl : plan.slots) {
+      const Problem& pb = sl.pb;
+      const int64_t f[] = {pb.n, pb.npf, pb.d, pb.nk, pb.nl, pb.kmax, pb.method, pb.model_type,
+                           pb.auto_feature, pb.max_steps, pb.num_desired};
+      for (int64_t v : f) key.push_back((size_t)v);
+    }
+    if (key != cache.key || old_arena != cache.arena.p || old_meta != cache.meta.p ||
+        !cache.first || !cache.rest) {
+      cache.drop_graphs();
+      cache.key.clear();
+      int rc = plan.capture(true, s, &cache.first);
+      if (!rc) rc = plan.capture(false, s, &cache.rest);
+      if (rc) {
+        cache.drop_graphs();
+        return rc;
+      }
+      cache.key = key;
+    }
+    for (int t = 0; t < max_steps; ++t) HIPCHK(hipGraphLaunch(t == 0 ? cache.first : cache.rest, s));
+  } else {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> marks;
+    for (int t = 0; t < max_steps; ++t) {
+      hipEvent_t a = tm.make(), b = tm.make();
+      int rc = plan.issue_step(t == 0, s, a, b);
+      if (rc) return rc;
+      marks.emplace_back(a, b);
+    }
+    HIPCHK(hipEventRecord(e2, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (auto& m : marks) ms_search += elapsed(m.first, m.second);
+  }
+  if (use_graph) {
+    HIPCHK(hipEventRecord(e2, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+
+  // ---- results
+  int64_t evals = 0, pruned_full = 0;
+  for (const Slot& sl : plan.slots) {
+    snn_problem& q = probs[sl.index];
+    const Problem& pb = sl.pb;
+    SelState hs{};
+    HIPCHK(hipMemcpy(&hs, at<SelState>(cache.arena, sl.state), sizeof(hs), hipMemcpyDeviceToHost));
+    if (q.selected_dimensions)
+      HIPCHK(hipMemcpy(q.selected_dimensions, at<int>(cache.arena, sl.sel), sizeof(int) * pb.d,
+                       hipMemcpyDeviceToHost));
+    if (q.selected_ordering)
+      HIPCHK(hipMemcpy(q.selected_ordering, at<int>(cache.arena, sl.ordering), sizeof(int) * pb.d,
+                       hipMemcpyDeviceToHost));
+    if (q.selected_errors)
+      HIPCHK(hipMemcpy(q.selected_errors, at<float>(cache.arena, sl.errors),
+                       sizeof(float) * pb.nl * pb.d, hipMemcpyDeviceToHost));
+    q.num_final = hs.num_final;
+    q.optimal_k = hs.best_k;
+    q.steps = hs.step;
+    if (hs.error) {
+      q.status = SNN_ESTATE;
+      if (first_err.empty())
+        first_err = "problem " + std::to_string(sl.index) +
+                    ": no candidate had a finite validation error";
+    }
+    for (int t = 0; t < hs.step; ++t) {
+      evals += (int64_t)(pb.d - t) * pb.n;
+      if (pruned && (t > 0 || pb.method == 0))
+        pruned_full += (int64_t)(pb.d - t) * pb.n * (pb.n - pb.npf);
+    }
+  }
+  if (summary) {
+    summary->steps = max_steps;
+    summary->ms_total = elapsed(e0, e2);
+    summary->ms_distance = elapsed(e0, e1);
+    summary->ms_search = ms_search;
+    summary->evals = evals;
+    if (pruned) {
+      unsigned long long done = 0;
+      HIPCHK(hipMemcpy(&done, at<unsigned long long>(cache.meta, plan.visited_off), sizeof(done),
+                       hipMemcpyDeviceToHost));
+      summary->pruned_point_evals = pruned_full - (int64_t)done;
+    }
+  }
+  if (!first_err.empty()) {
+    bool invalid = false;
+    for (int p = 0; p < num_problems; ++p) invalid |= probs[p].status != SNN_OK && probs[p].status != SNN_ESTATE;
+    return fail(invalid ? SNN_EINVAL : SNN_ESTATE, first_err);
+  }
+  return SNN_OK;
+}

@@ -47,6 +47,36 @@ struct Problem {
   int num_desired;
 };
 
+// One selection problem as the search kernels see it (passed by value to a
+// single-problem launch, or as an array to the persistent batched launch).
+struct SearchProblem {
+  const float* D;        // [n][n] current distances
+  const float* M;        // [n][n] max single-feature term (backward pruning)
+  const float* Xc;       // [n][cpad] candidate columns
+  const float* y;        // [n] labels
+  const int* kvals;      // [nk]
+  const SelState* st;    // device state
+  const int* seeds;      // [n][128] pruning seeds
+  float* P;              // [nk][cpad][n] predictions
+  unsigned long long* visited;  // point evaluations performed (pruned search)
+  int64_t n;
+  int64_t npf;           // points per fold
+  int cpad;
+  int nk;
+  int kmax;
+  int mult;              // +1 forward, -1 backward
+  int model_type;
+  int zero;              // D is known to be 0 (first forward step)
+};
+
+// Neighbour-list length compiled for a largest k value (-1: above 64).
+__host__ __device__ inline int search_list_size(int kmax) {
+  constexpr int sizes[] = {2, 4, 5, 8, 10, 15, 16, 20, 24, 32, 50, 64};
+  for (int v : sizes)
+    if (kmax <= v) return v;
+  return -1;
+}
+
 // Wave-uniform value (SGPR) of a value known to be uniform.
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 

@@ -1,0 +1,74 @@
+// Internal definitions shared by the host-side drivers (engine.hip: one
+// selection; batch.hip: many selections per launch).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/speedynn.h"
+#include "common.h"
+
+namespace snn {
+
+// Records the message of the calling thread's last error; returns code.
+int fail(int code, const std::string& msg);
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return ::snn::fail(e_ == hipErrorOutOfMemory ? SNN_ENOMEM : SNN_EDEVICE,              \
+                         std::string(#expr) + " failed: " + hipGetErrorString(e_));         \
+  } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t want) {
+    if (want <= bytes && p) return hipSuccess;
+    if (p) {
+      hipFree(p);
+      p = nullptr;
+      bytes = 0;
+    }
+    if (want == 0) want = 256;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) bytes = want;
+    return e;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// Checks one problem's arguments the way snn_select does and fills pb.
+int validate_problem(int64_t n, int d, const int* k_vals, int num_k, const int* loss_functions,
+                     int num_loss, const snn_params* prm, Problem* pb);
+
+// Cached batch launch state (graphs are rebuilt when the batch changes).
+struct BatchCache {
+  DevBuf arena, meta;
+  std::vector<size_t> key;  // shapes and arena addresses the graphs were built for
+  hipGraphExec_t first = nullptr, rest = nullptr;
+  void drop_graphs() {
+    if (first) hipGraphExecDestroy(first);
+    if (rest) hipGraphExecDestroy(rest);
+    first = rest = nullptr;
+  }
+};
+
+}  // namespace snn
+
+struct snn_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  snn::DevBuf X, y, XT, D, Dlo, Xc, P, err, val, bestk, cand, sel, ordering, errors, cand_err,
+      kvals, losses, state, scratch, seeds, visited, perm, key, M, med;
+  snn::BatchCache batch;
+};

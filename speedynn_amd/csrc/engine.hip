@@ -15,8 +15,11 @@
 
 #include "../../include/speedynn.h"
 #include "common.h"
+#include "engine.h"
 #include "kernels.h"
 
+using snn::DevBuf;
+using snn::fail;
 using snn::Problem;
 using snn::SelState;
 
@@ -24,67 +27,74 @@ namespace {
 
 thread_local std::string g_err = "no error";
 
-int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
-
-#define HIPCHK(expr)                                                                        \
-  do {                                                                                      \
-    hipError_t e_ = (expr);                                                                 \
-    if (e_ != hipSuccess)                                                                   \
-      return fail(e_ == hipErrorOutOfMemory ? SNN_ENOMEM : SNN_EDEVICE,                     \
-                  std::string(#expr) + " failed: " + hipGetErrorString(e_));               \
-  } while (0)
-
-// Grow-only device buffer.
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  hipError_t ensure(size_t want) {
-    if (want <= bytes && p) return hipSuccess;
-    if (p) {
-      hipFree(p);
-      p = nullptr;
-      bytes = 0;
-    }
-    if (want == 0) want = 256;
-    hipError_t e = hipMalloc(&p, want);
-    if (e == hipSuccess) bytes = want;
-    return e;
-  }
-  void release() {
-    if (p) hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  template <class T>
-  T* as() const { return static_cast<T*>(p); }
-};
-
-}  // namespace
-
-struct snn_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  DevBuf X, y, XT, D, Dlo, Xc, P, err, val, bestk, cand, sel, ordering, errors, cand_err, kvals,
-      losses, state, scratch, seeds, visited, perm, key, M;
-};
-
-namespace {
-
 int validate_losses(const int* loss_functions, int num_loss) {
   if (num_loss < 1) return fail(SNN_EINVAL, "at least one loss function is required");
   for (int l = 0; l < num_loss; ++l) {
     const int c = loss_functions[l];
     if (c < 0 || c > 8) return fail(SNN_EINVAL, "unknown loss function code " + std::to_string(c));
-    if (c == snn::kMad || c == snn::kMadNorm || c == snn::kRmseNormMadNormStd ||
-        c == snn::kRmseNormMadNormGalaxies)
-      return fail(SNN_ENOTSUP, "median-based losses (mad, mad_norm and their composites) are "
-                               "not implemented on the device yet");
   }
   return SNN_OK;
 }
+
+}  // namespace
+
+namespace snn {
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int validate_problem(int64_t n, int d, const int* k_vals, int num_k, const int* loss_functions,
+                     int num_loss, const snn_params* prm, Problem* out) {
+  if (!prm) return fail(SNN_EINVAL, "null params");
+  if (n < 2 || d < 1) return fail(SNN_EINVAL, "need n >= 2 points and d >= 1 features");
+  if (n > (int64_t)INT32_MAX) return fail(SNN_EINVAL, "n exceeds the 32-bit neighbour index");
+  if (prm->method != 0 && prm->method != 1) return fail(SNN_EINVAL, "method must be 0 or 1");
+  if (prm->model_type != 0 && prm->model_type != 1)
+    return fail(SNN_EINVAL, "model_type must be 0 (regression) or 1 (classification)");
+  if (prm->form != SNN_FORM_PLAIN && prm->form != SNN_FORM_PRECISION)
+    return fail(SNN_EINVAL, "form must be SNN_FORM_PLAIN or SNN_FORM_PRECISION");
+  if (prm->k_fold < 1 || prm->k_fold > n)
+    return fail(SNN_EINVAL, "k_fold must be in [1, n] (k_fold > n divides by zero in the reference)");
+  if (num_k < 1 || !k_vals) return fail(SNN_EINVAL, "at least one k value is required");
+  int kmax = 0;
+  for (int m = 0; m < num_k; ++m) {
+    if (k_vals[m] < 1) return fail(SNN_EINVAL, "k values must be >= 1");
+    kmax = k_vals[m] > kmax ? k_vals[m] : kmax;
+  }
+  if (search_list_size(kmax) < 0) return fail(SNN_ENOTSUP, "k values above 64 are not supported");
+  if (!loss_functions) return fail(SNN_EINVAL, "null loss function list");
+  int rc = validate_losses(loss_functions, num_loss);
+  if (rc) return rc;
+  if (!prm->auto_feature && prm->num_features < 1)
+    return fail(SNN_EINVAL, "num_features must be > 0");
+  if (!prm->auto_feature && prm->num_features > d)
+    return fail(SNN_EINVAL, "num_features must be <= the number of features");
+
+  Problem pb{};
+  pb.n = n;
+  pb.npf = n / prm->k_fold;
+  pb.d = d;
+  pb.cpad = (d + 63) / 64 * 64;
+  if (n * (int64_t)pb.cpad >= ((int64_t)1 << 31))
+    return fail(SNN_ENOTSUP, "n * padded d must stay below 2^31 (32-bit gather offsets)");
+  pb.nk = num_k;
+  pb.nl = num_loss;
+  pb.kmax = kmax;
+  pb.method = prm->method;
+  pb.model_type = prm->model_type;
+  pb.auto_feature = prm->auto_feature ? 1 : 0;
+  const bool fwd = prm->method == 1;
+  pb.num_desired = prm->auto_feature ? (fwd ? d : 0) : prm->num_features;
+  pb.max_steps = fwd ? pb.num_desired : d - pb.num_desired;
+  *out = pb;
+  return SNN_OK;
+}
+
+}  // namespace snn
+
+namespace {
 
 struct Timing {
   std::vector<hipEvent_t> ev;
@@ -110,44 +120,11 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
                 const snn_params* prm, int* selected_dimensions, int* selected_ordering,
                 float* selected_errors, float* candidate_errors, snn_summary* summary) {
   if (!ctx) return fail(SNN_EINVAL, "null context");
-  if (!prm) return fail(SNN_EINVAL, "null params");
-  if (n < 2 || d < 1) return fail(SNN_EINVAL, "need n >= 2 points and d >= 1 features");
-  if (n > (int64_t)INT32_MAX) return fail(SNN_EINVAL, "n exceeds the 32-bit neighbour index");
-  if (prm->method != 0 && prm->method != 1) return fail(SNN_EINVAL, "method must be 0 or 1");
-  if (prm->model_type != 0 && prm->model_type != 1)
-    return fail(SNN_EINVAL, "model_type must be 0 (regression) or 1 (classification)");
-  if (prm->k_fold < 1 || prm->k_fold > n)
-    return fail(SNN_EINVAL, "k_fold must be in [1, n] (k_fold > n divides by zero in the reference)");
-  if (num_k < 1 || !k_vals) return fail(SNN_EINVAL, "at least one k value is required");
-  int kmax = 0;
-  for (int m = 0; m < num_k; ++m) {
-    if (k_vals[m] < 1) return fail(SNN_EINVAL, "k values must be >= 1");
-    kmax = k_vals[m] > kmax ? k_vals[m] : kmax;
-  }
-  if (snn::search_list_size(kmax) < 0) return fail(SNN_ENOTSUP, "k values above 64 are not supported");
-  int rc = validate_losses(loss_functions, num_loss);
-  if (rc) return rc;
-  if (!prm->auto_feature && prm->num_features < 1)
-    return fail(SNN_EINVAL, "num_features must be > 0");
-  if (!prm->auto_feature && prm->num_features > d)
-    return fail(SNN_EINVAL, "num_features must be <= the number of features");
-
   Problem pb{};
-  pb.n = n;
-  pb.npf = n / prm->k_fold;
-  pb.d = d;
-  pb.cpad = (d + 63) / 64 * 64;
-  if (n * (int64_t)pb.cpad >= ((int64_t)1 << 31))
-    return fail(SNN_ENOTSUP, "n * padded d must stay below 2^31 (32-bit gather offsets)");
-  pb.nk = num_k;
-  pb.nl = num_loss;
-  pb.kmax = kmax;
-  pb.method = prm->method;
-  pb.model_type = prm->model_type;
-  pb.auto_feature = prm->auto_feature ? 1 : 0;
+  int rc = snn::validate_problem(n, d, k_vals, num_k, loss_functions, num_loss, prm, &pb);
+  if (rc) return rc;
+  const int kmax = pb.kmax;
   const bool fwd = prm->method == 1;
-  pb.num_desired = prm->auto_feature ? (fwd ? d : 0) : prm->num_features;
-  pb.max_steps = fwd ? pb.num_desired : d - pb.num_desired;
   const bool comp = prm->form == SNN_FORM_PRECISION;
 
   HIPCHK(hipSetDevice(ctx->device));
@@ -176,7 +153,12 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   if (pruned) {
     HIPCHK(ctx->seeds.ensure(sizeof(int) * (size_t)n * 128));
     HIPCHK(ctx->visited.ensure(sizeof(unsigned long long)));
-    HIPCHK(hipMemsetAsync(ctx->visited.p, 0, sizeof(unsigned long long), ctx->stream));
+    HIPCHK(snn::launch_fill_u64(ctx->visited.as<unsigned long long>(), 1, ctx->stream));
+  }
+  float* med = nullptr;  // median losses: sort space for NaN-holding error vectors
+  if (snn::needs_median_scratch(loss_functions, num_loss)) {
+    HIPCHK(ctx->med.ensure(sizeof(float) * 2 * (size_t)n * pb.cpad * num_k));
+    med = ctx->med.as<float>();
   }
   float* cand_err = nullptr;
   if (candidate_errors) {
@@ -214,7 +196,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   }
   HIPCHK(hipEventRecord(e1, s));
 
-  snn::SearchLaunch sl{};
+  snn::SearchProblem sl{};
   sl.D = D;
   sl.Xc = ctx->Xc.as<float>();
   sl.y = dy;
@@ -251,7 +233,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     }
     HIPCHK(hipEventRecord(c, s));
     HIPCHK(snn::launch_loss(ctx->P.as<float>(), dy, ctx->losses.as<int>(), st,
-                            ctx->err.as<float>(), n, pb.cpad, num_k, num_loss, d, s));
+                            ctx->err.as<float>(), n, pb.cpad, num_k, num_loss, d, med, s));
     HIPCHK(snn::launch_select(st, pb, ctx->kvals.as<int>(), ctx->err.as<float>(),
                               ctx->cand.as<int>(), ctx->sel.as<int>(), ctx->ordering.as<int>(),
                               ctx->errors.as<float>(), ctx->val.as<float>(), ctx->bestk.as<int>(),
@@ -392,8 +374,11 @@ void snn_destroy(snn_ctx* c) {
   DevBuf* bufs[] = {&c->X, &c->y, &c->XT, &c->D, &c->Dlo, &c->Xc, &c->P, &c->err, &c->val,
                     &c->bestk, &c->cand, &c->sel, &c->ordering, &c->errors, &c->cand_err,
                     &c->kvals, &c->losses, &c->state, &c->scratch, &c->seeds,
-                    &c->visited, &c->perm, &c->key, &c->M};
+                    &c->visited, &c->perm, &c->key, &c->M, &c->med};
   for (DevBuf* b : bufs) b->release();
+  c->batch.drop_graphs();
+  c->batch.arena.release();
+  c->batch.meta.release();
   hipStreamDestroy(c->stream);
   delete c;
 }
@@ -476,13 +461,14 @@ int snn_loss(snn_ctx* ctx, const float* labels, const float* predictions, int64_
   int rc = validate_losses(&loss_function, 1);
   if (rc) return rc;
   HIPCHK(hipSetDevice(ctx->device));
-  HIPCHK(ctx->scratch.ensure(sizeof(float) * (size_t)(2 * n + 64)));
+  HIPCHK(ctx->scratch.ensure(sizeof(float) * (size_t)(4 * n + 64)));
   float* dy = ctx->scratch.as<float>();
   float* dp = dy + n;
   float* dout = dp + n;
+  float* dmed = dout + 64;
   HIPCHK(hipMemcpyAsync(dy, labels, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(dp, predictions, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(snn::launch_loss_single(dy, dp, n, loss_function, dout, ctx->stream));
+  HIPCHK(snn::launch_loss_single(dy, dp, n, loss_function, dout, dmed, ctx->stream));
   HIPCHK(hipMemcpyAsync(out, dout, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return SNN_OK;

@@ -1,32 +1,11 @@
-// Host-side launchers of the engine's kernels (kernels.hip).
+// Host-side launchers of the engine's kernels (kernels.hip, search.hip).
 #pragma once
 #include "common.h"
 
 namespace snn {
 
-struct SearchLaunch {
-  const float* D;
-  const float* Xc;
-  const float* y;
-  const int* kvals;
-  const SelState* st;
-  float* P;
-  const int* seeds;  // pruned search: [n][128] bound seeds per query
-  const float* M;    // backward pruned search: [n][n] max single-feature term
-  unsigned long long* visited;  // pruned search: (query, point) pairs examined
-  int64_t n;
-  int64_t npf;
-  int cpad;
-  int nk;
-  int kmax;
-  int mult;
-  int model_type;
-  int zero;       // dense forward search with D known to be 0
-};
-
-int search_list_size(int kmax);
-
 hipError_t launch_fill(float* p, int64_t count, float v, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, int64_t count, hipStream_t s);
 hipError_t launch_transpose(const float* X, float* XT, int64_t n, int d, hipStream_t s);
 hipError_t launch_gram_exact(const float* XT, float* D, float* Dlo, float* M, int64_t n, int d,
                              hipStream_t s);
@@ -34,15 +13,19 @@ hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const
                          int f, int mult, hipStream_t s);
 hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* Xc, int64_t n,
                           int d, int cpad, hipStream_t s);
-hipError_t launch_search_dense(const SearchLaunch& a, hipStream_t s);
-hipError_t launch_search_pruned(const SearchLaunch& a, hipStream_t s);
+hipError_t launch_search_dense(const SearchProblem& a, hipStream_t s);
+hipError_t launch_search_pruned(const SearchProblem& a, hipStream_t s);
 hipError_t launch_seed_scan(const float* D, const SelState* st, int* seeds, int64_t n,
                             int64_t npf, hipStream_t s);
+// scratch: median losses with NaN errors sort a copy there, 2n floats per
+// (k value, candidate slot) -- nk * cpad * 2n for launch_loss, 2n for
+// launch_loss_single; may be null when needs_median_scratch() is false.
 hipError_t launch_loss(const float* P, const float* y, const int* losses, const SelState* st,
                        float* err, int64_t n, int cpad, int nk, int nl, int max_cand,
-                       hipStream_t s);
+                       float* scratch, hipStream_t s);
 hipError_t launch_loss_single(const float* y, const float* p, int64_t n, int code, float* out,
-                              hipStream_t s);
+                              float* scratch, hipStream_t s);
+bool needs_median_scratch(const int* codes, int count);
 hipError_t launch_select(SelState* st, const Problem& pb, const int* kvals, const float* err,
                          int* cand, int* sel, int* ordering, float* errors, float* val,
                          int* bestk, float* cand_err, const int* perm, hipStream_t s);
@@ -50,5 +33,17 @@ hipError_t launch_feature_order(const float* XT, int64_t n, int d, double* key, 
                                 bool by_scale, hipStream_t s);
 hipError_t launch_init_state(SelState* st, const Problem& pb, int* cand, int* sel, int* ordering,
                              float* errors, const int* perm, hipStream_t s);
+
+// Persistent search over many problems (device array `probs`, item prefix
+// sums `start[P+1]`, work counter).  mode 0: dense, 1: dense with D = 0
+// (first forward step), 2: pruned; every problem of a launch has direction
+// `mult` (+1 forward, -1 backward).  List sizes up to batch_max_list_size().
+int batch_max_list_size();
+int64_t batch_dense_items(const SearchProblem& a);
+int64_t batch_pruned_items(const SearchProblem& a);
+int batch_grid(int mode);  // persistent grid size for a mode on the current device
+hipError_t launch_batch_search(const SearchProblem* probs, const int64_t* start, int P,
+                               unsigned long long* counter, int mode, int mult, int grid,
+                               hipStream_t s);
 
 }  // namespace snn

@@ -14,6 +14,7 @@
 // kernel calls fmaf() on purpose.
 #include "common.h"
 #include "kernels.h"
+#include "qsort_order.h"
 
 namespace snn {
 
@@ -26,6 +27,11 @@ __global__ void k_fill(float* __restrict__ p, int64_t count, float v) {
   float4* p4 = reinterpret_cast<float4*>(p);
   for (int64_t t = i; t < n4; t += stride) p4[t] = make_float4(v, v, v, v);
   for (int64_t t = (n4 << 2) + i; t < count; t += stride) p[t] = v;
+}
+
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, int64_t count) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += stride) p[t] = 0;
 }
 
 // XT[f][j] = X[j][f], 32x32 tiles through LDS.
@@ -256,8 +262,128 @@ __device__ __forceinline__ int zero_one_err(float yv, float pv) {
   return ((int)v != 0) ? 1 : 0;
 }
 
+// loss_functions.c:41-50
+__device__ float loss_rmse(const float* __restrict__ y, const float* __restrict__ p, int64_t n) {
+  const float s = seq_sum<false>(n, [&](int64_t q) {
+    const float t = y[q] - p[q];
+    return t * t;
+  });
+  return (float)sqrt((double)(s / (float)n));
+}
+
+// loss_functions.c:52-62 (summed from the last point down)
+__device__ float loss_rmse_norm(const float* __restrict__ y, const float* __restrict__ p,
+                                int64_t n) {
+  const float s = seq_sum<true>(n, [&](int64_t q) {
+    const float tv = y[q];
+    const float t = (float)((double)(tv - p[q]) / (1.0 + (double)tv));
+    return t * t;
+  });
+  return (float)sqrt((double)(s / (float)n));
+}
+
+// loss_functions.c:129-145, keeping the assignment at :141 (only the last
+// squared deviation survives)
+__device__ float loss_std(const float* __restrict__ y, const float* __restrict__ p, int64_t n) {
+  float mean = seq_sum<false>(n, [&](int64_t q) { return y[q] - p[q]; });
+  mean = mean / (float)n;
+  const float t = (y[n - 1] - p[n - 1]) - mean;
+  const float s = (t * t) / (float)(n - 1);
+  return (float)sqrt((double)s);
+}
+
+// Order statistic `kth` (0-based) of value(i), i < n, one wave: radix select
+// over a total order of the float bits, 8 bits per pass, histograms in LDS.
+// The sorted position the reference reads (qsort, loss_functions.c:173-183)
+// holds this value; only +0 and -0 could tie without being equal bits, and
+// both callers are indifferent to the sign of a zero median.
+template <class Value>
+__device__ float wave_select(int64_t n, int64_t kth, unsigned int* hist, Value value) {
+  const int lane = threadIdx.x & 63;
+  auto key = [](float v) {
+    const unsigned int b = __float_as_uint(v);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  };
+  unsigned int prefix = 0, mask = 0;
+  int64_t rank = kth;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    for (int b = lane; b < 256; b += 64) hist[b] = 0;
+    __syncthreads();
+    for (int64_t i = lane; i < n; i += 64) {
+      const unsigned int k = key(value(i));
+      if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    int64_t cum = 0;
+    unsigned int sel = 255;
+    for (int b = 0; b < 256; ++b) {  // every lane scans the same LDS histogram
+      const int64_t h = hist[b];
+      if (cum + h > rank) {
+        sel = (unsigned int)b;
+        break;
+      }
+      cum += h;
+    }
+    rank -= cum;
+    prefix |= sel << shift;
+    mask |= 255u << shift;
+    __syncthreads();
+  }
+  const unsigned int b = (prefix & 0x80000000u) ? (prefix & 0x7fffffffu) : ~prefix;
+  return __uint_as_float(b);
+}
+
+// Element n/2 of buf[0, n) in the order the reference's qsort leaves it
+// (qsort_order.h), tmp of n floats; the block is one wave.  Lanes sort the 64
+// subtrees at depth 6 of the merge recursion, one each, then merge level by
+// level; the barriers also make each level's global-memory writes visible to
+// the next.
+__device__ float reference_median(float* buf, float* tmp, int64_t n) {
+  constexpr int kDepth = 6;
+  const int lane = threadIdx.x & 63;
+  sort_subtree(buf, tmp, n, kDepth, lane);
+  __syncthreads();
+  for (int t = kDepth - 1; t >= 0; --t) {
+    if (lane < (1 << t)) merge_level_node(buf, tmp, n, t, lane);
+    __syncthreads();
+  }
+  const float m = buf[n / 2];
+  __syncthreads();
+  return m;
+}
+
+// Median (element n/2 of the reference's sorted copy) of value(i), i < n:
+// radix select unless some value is NaN, else the reference's sort in
+// scratch (2n floats).
+template <class Value>
+__device__ float median_of(int64_t n, unsigned int* hist, float* scratch, Value value) {
+  const int lane = threadIdx.x & 63;
+  bool nan = false;
+  for (int64_t i = lane; i < n; i += 64) {
+    const float v = value(i);
+    nan |= v != v;
+  }
+  if (!__any(nan) || !scratch) return wave_select(n, n / 2, hist, value);
+  float* buf = scratch;
+  for (int64_t i = lane; i < n; i += 64) buf[i] = value(i);
+  __syncthreads();
+  return reference_median(buf, scratch + n, n);
+}
+
+// loss_functions.c:67-111: median of |e - median(e)|, e = y - p (mad) or
+// (y - p) / (1 + y) rounded to float (mad_norm).  scratch: 2n floats.
+__device__ float loss_mad(const float* __restrict__ y, const float* __restrict__ p, int64_t n,
+                          bool norm, unsigned int* hist, float* scratch) {
+  auto err = [&](int64_t q) {
+    const float tv = y[q];
+    return norm ? (float)((double)(tv - p[q]) / (1.0 + (double)tv)) : tv - p[q];
+  };
+  const float m1 = median_of(n, hist, scratch, err);
+  return median_of(n, hist, scratch, [&](int64_t q) { return fabsf(err(q) - m1); });
+}
+
 __device__ float loss_value(int code, const float* __restrict__ y, const float* __restrict__ p,
-                            int64_t n) {
+                            int64_t n, unsigned int* hist, float* scratch) {
   const int lane = threadIdx.x & 63;
   switch (code) {
     case kZeroOne: {
@@ -266,33 +392,21 @@ __device__ float loss_value(int code, const float* __restrict__ y, const float* 
       for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
       return (float)cnt / (float)n;
     }
-    case kRmse: {
-      const float s = seq_sum<false>(n, [&](int64_t q) {
-        const float t = y[q] - p[q];
-        return t * t;
-      });
-      return (float)sqrt((double)(s / (float)n));
+    case kRmse: return loss_rmse(y, p, n);
+    case kRmseNorm: return loss_rmse_norm(y, p, n);
+    case kMad: return loss_mad(y, p, n, false, hist, scratch);
+    case kMadNorm: return loss_mad(y, p, n, true, hist, scratch);
+    case kStd: return loss_std(y, p, n);
+    case kRmseNormMadNormStd: {  // loss_functions.c:147-152, summed in double
+      const float a = loss_rmse_norm(y, p, n), b = loss_mad(y, p, n, true, hist, scratch);
+      const float c = loss_std(y, p, n);
+      return (float)((double)a / 0.19 + (double)b / 0.041 + (double)c / 0.16);
     }
-    case kRmseNorm: {
-      const float s = seq_sum<true>(n, [&](int64_t q) {
-        const float tv = y[q];
-        const float t = (float)((double)(tv - p[q]) / (1.0 + (double)tv));
-        return t * t;
-      });
-      return (float)sqrt((double)(s / (float)n));
+    case kRmseNormMadNormGalaxies: {  // loss_functions.c:154-159
+      const float a = loss_rmse_norm(y, p, n), b = loss_mad(y, p, n, true, hist, scratch);
+      return (float)((double)a / 0.019 + (double)b / 0.009);
     }
-    case kStd: {
-      float mean = seq_sum<false>(n, [&](int64_t q) { return y[q] - p[q]; });
-      mean = mean / (float)n;
-      const float t = (y[n - 1] - p[n - 1]) - mean;
-      const float s = (t * t) / (float)(n - 1);
-      return (float)sqrt((double)s);
-    }
-    case kRmseStd: {
-      const float a = loss_value(kRmse, y, p, n);
-      const float b = loss_value(kStd, y, p, n);
-      return a + b;
-    }
+    case kRmseStd: return loss_rmse(y, p, n) + loss_std(y, p, n);  // loss_functions.c:33-34
     default:
       return __int_as_float(0x7fc00000);
   }
@@ -304,22 +418,27 @@ __global__ __launch_bounds__(64) void k_loss(const float* __restrict__ P,
                                              const int* __restrict__ losses,
                                              const SelState* __restrict__ st,
                                              float* __restrict__ err, int64_t n, int cpad, int nk,
-                                             int nl) {
+                                             int nl, float* __restrict__ scratch) {
+  __shared__ unsigned int hist[256];
   if (st->stopped) return;
   const int c = blockIdx.x, m = blockIdx.y;
   if (c >= st->ncand) return;
-  const float* p = P + ((int64_t)m * cpad + c) * n;
+  const int64_t slot = (int64_t)m * cpad + c;
+  const float* p = P + slot * n;
+  float* scr = scratch ? scratch + slot * 2 * n : nullptr;  // median losses only
   for (int l = 0; l < nl; ++l) {
-    const float e = loss_value(losses[l], y, p, n);
+    const float e = loss_value(losses[l], y, p, n, hist, scr);
     if (threadIdx.x == 0) err[((int64_t)l * nk + m) * cpad + c] = e;
   }
 }
 
-// Loss of one prediction vector (get_loss_all).
+// Loss of one prediction vector (get_loss_all); scratch: 2n floats.
 __global__ __launch_bounds__(64) void k_loss_single(const float* __restrict__ y,
                                                     const float* __restrict__ p, int64_t n,
-                                                    int code, float* __restrict__ out) {
-  const float e = loss_value(code, y, p, n);
+                                                    int code, float* __restrict__ out,
+                                                    float* __restrict__ scratch) {
+  __shared__ unsigned int hist[256];
+  const float e = loss_value(code, y, p, n, hist, scratch);
   if (threadIdx.x == 0) *out = e;
 }
 
@@ -482,6 +601,11 @@ static inline int grid_1d(int64_t work, int block, int cap) {
   return (int)(g > cap ? cap : g);
 }
 
+hipError_t launch_fill_u64(unsigned long long* p, int64_t count, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(grid_1d(count, 256, 1024)), dim3(256), 0, s, p, count);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill(float* p, int64_t count, float v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill, dim3(grid_1d(count / 4, 256, 16384)), dim3(256), 0, s, p, count, v);
   return hipGetLastError();
@@ -527,16 +651,24 @@ hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* 
 
 hipError_t launch_loss(const float* P, const float* y, const int* losses, const SelState* st,
                        float* err, int64_t n, int cpad, int nk, int nl, int max_cand,
-                       hipStream_t s) {
+                       float* scratch, hipStream_t s) {
   dim3 g((unsigned)max_cand, (unsigned)nk);
-  hipLaunchKernelGGL(k_loss, g, dim3(64), 0, s, P, y, losses, st, err, n, cpad, nk, nl);
+  hipLaunchKernelGGL(k_loss, g, dim3(64), 0, s, P, y, losses, st, err, n, cpad, nk, nl, scratch);
   return hipGetLastError();
 }
 
 hipError_t launch_loss_single(const float* y, const float* p, int64_t n, int code, float* out,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(k_loss_single, dim3(1), dim3(64), 0, s, y, p, n, code, out);
+                              float* scratch, hipStream_t s) {
+  hipLaunchKernelGGL(k_loss_single, dim3(1), dim3(64), 0, s, y, p, n, code, out, scratch);
   return hipGetLastError();
+}
+
+bool needs_median_scratch(const int* codes, int count) {
+  for (int l = 0; l < count; ++l)
+    if (codes[l] == kMad || codes[l] == kMadNorm || codes[l] == kRmseNormMadNormStd ||
+        codes[l] == kRmseNormMadNormGalaxies)
+      return true;
+  return false;
 }
 
 hipError_t launch_select(SelState* st, const Problem& pb, const int* kvals, const float* err,

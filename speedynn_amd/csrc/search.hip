@@ -1,5 +1,8 @@
 // Candidate search kernels (c) of the kNN selection engine, gfx950 wave64.
-// See kernels.hip for the data layout.
+// See kernels.hip for the data layout.  The work of one (query block,
+// candidate block) or one (query, candidate block) pair is a device function,
+// called by the single-problem kernels and by the persistent kernels that
+// serve many problems from one work queue.
 #include "common.h"
 #include "kernels.h"
 
@@ -63,47 +66,39 @@ __device__ __forceinline__ float predict(const float* lab, int k, int model_type
 // wave's LDS column `lab`, then the prediction of every k value is written to
 // P[m][c][q].
 template <int K>
-__device__ __forceinline__ void finish_query(const int (&Li)[K], float* lab,
-                                             const float* __restrict__ y,
-                                             const int* __restrict__ kvals, int nk,
-                                             int model_type, float* __restrict__ P, int64_t q,
-                                             int c, int cpad, int64_t n, bool live) {
+__device__ __forceinline__ void finish_query(const SearchProblem& a, const int (&Li)[K],
+                                             float* lab, int64_t q, int c, bool live) {
 #pragma unroll
-  for (int m = 0; m < K; ++m) lab[m * 64] = y[Li[m]];
-  for (int m = 0; m < nk; ++m) {
-    const int k = kvals[m];
-    const float pred = predict(lab, k, model_type);
-    if (live) P[((int64_t)m * cpad + c) * n + q] = pred;
+  for (int m = 0; m < K; ++m) lab[m * 64] = a.y[Li[m]];
+  for (int m = 0; m < a.nk; ++m) {
+    const float pred = predict(lab, a.kvals[m], a.model_type);
+    if (live) a.P[((int64_t)m * a.cpad + c) * a.n + q] = pred;
   }
 }
 
-// Dense search, one lane per (query, candidate): a wave owns QW queries and
-// the 64 candidates of block blockIdx.y; a block owns WAVES*QW consecutive
-// queries.  Per step of 64 points the block stages the candidates' columns
+// Dense search of one item: queries [qb*WAVES*QW, +WAVES*QW) x candidate
+// block cb, one lane per (query, candidate): a wave owns QW queries and the 64
+// candidates.  Per step of 64 points the block stages the candidates' columns
 // Xc[j0:j0+64][cb*64:+64] in LDS (double-buffered), so each staged value
 // serves WAVES*QW queries; the query's D row is wave-uniform and read with
 // scalar loads.  Per point: d = fl(D_qj + mult*fl((x_qf - x_jf)^2))
 // (speedynn/lib/cpu.c:142-144), points in the query's fold are skipped
 // (cpu.c:139: LOO when k_fold == n).  ZERO: D is known to be 0 (first forward
-// step), so d = fl((x_qf - x_jf)^2) exactly and D is not read.
+// step), so d = fl((x_qf - x_jf)^2) exactly and D is not read.  Called by the
+// whole block; returns early only block-uniformly.
 template <int K, int QW, int WAVES, int MULT, bool ZERO>
-__global__ __launch_bounds__(64 * WAVES) void k_search_dense(
-    const float* __restrict__ D, const float* __restrict__ Xc, const float* __restrict__ y,
-    const int* __restrict__ kvals, const SelState* __restrict__ st, float* __restrict__ P,
-    int64_t n, int64_t npf, int cpad, int nk, int model_type) {
+__device__ __forceinline__ void dense_block(const SearchProblem& a, float* lds, int64_t qb,
+                                            int cb) {
   constexpr int J = 64;
-  constexpr int LDS_SCAN = 2 * J * 64;
-  constexpr int LDS_EPI = WAVES * K * 64;
-  __shared__ float lds[LDS_SCAN > LDS_EPI ? LDS_SCAN : LDS_EPI];
-
-  if (st->stopped) return;
-  const int ncand = st->ncand;
-  const int cb = blockIdx.y;
+  if (a.st->stopped) return;
+  const int ncand = a.st->ncand;
   if (cb * 64 >= ncand) return;
+  const int64_t n = a.n, npf = a.npf;
+  const int cpad = a.cpad;
   const int lane = threadIdx.x & 63;
   const int wave = uniform(threadIdx.x >> 6);
   const int c = cb * 64 + lane;
-  const int64_t qbase = ((int64_t)blockIdx.x * WAVES + wave) * QW;
+  const int64_t qbase = (qb * WAVES + wave) * QW;
 
   float xq[QW];
   float Ld[QW][K];
@@ -113,9 +108,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_search_dense(
 #pragma unroll
   for (int qi = 0; qi < QW; ++qi) {
     const int64_t q = qbase + qi;
-    // queries past n (last block) scan row n-1 and never store a result
-    Drow[qi] = D + (q < n ? q : n - 1) * n;
-    xq[qi] = q < n ? Xc[q * cpad + c] : 0.0f;
+    // queries past n (last item) scan row n-1 and never store a result
+    Drow[qi] = a.D + (q < n ? q : n - 1) * n;
+    xq[qi] = q < n ? a.Xc[q * cpad + c] : 0.0f;
     const int64_t fold = q / npf;
     flo[qi] = q < n ? fold * npf : n;
     fhi[qi] = q < n ? (fold + 1) * npf : n;
@@ -126,7 +121,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_search_dense(
     }
   }
 
-  const float* __restrict__ Xblk = Xc + cb * 64;
+  const float* __restrict__ Xblk = a.Xc + cb * 64;
   auto stage = [&](int buf, int64_t j0) {
     float* dst = lds + buf * (J * 64);
     for (int t = threadIdx.x; t < J * 16; t += 64 * WAVES) {
@@ -138,6 +133,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_search_dense(
     }
   };
 
+  __syncthreads();  // the previous item of a persistent block may still read lds
   stage(0, 0);
   __syncthreads();
   int buf = 0;
@@ -195,7 +191,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_search_dense(
   for (int qi = 0; qi < QW; ++qi) {
     const int64_t q = qbase + qi;
     if (q >= n) break;
-    finish_query<K>(Li[qi], lab, y, kvals, nk, model_type, P, q, c, cpad, n, live);
+    finish_query<K>(a, Li[qi], lab, q, c, live);
   }
 }
 
@@ -203,42 +199,6 @@ __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
   return v;
-}
-
-// Seeds of the pruning bound: for each query, lane L of one wave keeps the two
-// smallest eligible D_qj over the residue class j = L (mod 64), ties to the
-// lower index; 128 distinct points, -1 where a class has fewer than two.
-__global__ __launch_bounds__(256) void k_seed_scan(const float* __restrict__ D,
-                                                   const SelState* __restrict__ st,
-                                                   int* __restrict__ seeds, int64_t n,
-                                                   int64_t npf) {
-  if (st->stopped) return;
-  const int lane = threadIdx.x & 63;
-  const int64_t q = (int64_t)blockIdx.x * 4 + uniform(threadIdx.x >> 6);
-  if (q >= n) return;
-  const float* __restrict__ Dq = D + q * n;
-  const int64_t flo = (q / npf) * npf;
-  const int64_t fhi = flo + npf < n ? flo + npf : n;
-  const float kInf = __int_as_float(0x7f800000);
-  float b0 = kInf, b1 = kInf;
-  int i0 = -1, i1 = -1;
-  for (int64_t j = lane; j < n; j += 64) {
-    if (j >= flo && j < fhi) continue;
-    const float v = Dq[j];
-    if (v < b1) {
-      if (v < b0) {
-        b1 = b0;
-        i1 = i0;
-        b0 = v;
-        i0 = (int)j;
-      } else {
-        b1 = v;
-        i1 = (int)j;
-      }
-    }
-  }
-  seeds[q * 128 + lane] = i0;
-  seeds[q * 128 + 64 + lane] = i1;
 }
 
 // Values-only sorted insertion (the bound needs the K-th value, not indices).
@@ -249,9 +209,8 @@ __device__ __forceinline__ void kth_insert(float (&B)[K], float v) {
   B[0] = v < B[0] ? v : B[0];
 }
 
-
-// Pruned search with an exact bound (precision form: forward steps after the
-// first, every backward step).
+// Pruned search of one (query q, candidate block cb) item by one wave
+// (precision form: forward steps after the first, every backward step).
 //
 // Bound.  For a candidate f, the K-th smallest d_qjf over any K distinct
 // eligible points is an upper bound tau_f of its final K-th distance; here it
@@ -261,58 +220,35 @@ __device__ __forceinline__ void kth_insert(float (&B)[K], float v) {
 //   backward (MULT = -1): M_qj >= fl(t^2) for every feature (max single-
 //     feature term, computed with the full matrix), so d = fl(D - fl(t^2)) >=
 //     fl(D - M) = lb by monotone rounding.
-// A point with lb > T_w = max over wave w's candidates of tau_f can enter no
-// lane's final top-K and is skipped; T_w tightens to the lists' own K-th
+// A point with lb > T = max over the wave's candidates of tau_f can enter no
+// lane's final top-K and is skipped; T tightens to the lists' own K-th
 // distance as they fill.  Points are still offered in ascending index order,
 // so every list equals the dense search's bit for bit.
 //
-// Layout.  One block = one query; wave w = candidate slots [64w', 64w'+64)
-// with w' = blockIdx.y*W + w, so the D (and M) row comes from HBM once per
-// block and the other waves hit in cache.  Per 64 points a wave takes the
-// ballot of lb <= T_w outside the query's fold (the lower bounds of the next
-// four chunks are already in flight), then for each such point, ascending,
-// gathers the candidates' values Xc[j][64w'+lane] (256 B, coalesced) and
-// evaluates d.  No block barriers.  Forward tests use a screen fma(t,t,D) against the
-// K-th distance widened by 2^-20 (plus 1e-36 for the subnormal range):
-// |fl(D+fl(t*t)) - fma(t,t,D)| <= 3.01*2^-24*(D+t*t) for D, t*t >= 0, so a
-// rejected point cannot pass the exact test; passes recompute d in the
+// Per 64 points the wave takes the ballot of lb <= T outside the query's fold
+// (the D/M values of the next four chunks are already in flight), then for
+// each such point, ascending, gathers the candidates' values Xc[j][64cb+lane]
+// (256 B, coalesced) and evaluates d.  Forward tests use a screen fma(t,t,D)
+// against the K-th distance widened by 2^-20 (plus 1e-36 for the subnormal
+// range): |fl(D+fl(t*t)) - fma(t,t,D)| <= 3.01*2^-24*(D+t*t) for D, t*t >= 0,
+// so a rejected point cannot pass the exact test; passes recompute d in the
 // reference's order.  Backward tests are exact (no screen: the fma error is
-// not bounded relative to D - t^2).
+// not bounded relative to D - t^2).  No barriers.
 template <int K, int MULT>
-__global__ __launch_bounds__(256) void k_search_pruned(
-    const float* __restrict__ D, const float* __restrict__ M, const float* __restrict__ Xc,
-    const float* __restrict__ y, const int* __restrict__ kvals, const SelState* __restrict__ st,
-    const int* __restrict__ seeds, float* __restrict__ P,
-    unsigned long long* __restrict__ visited, int64_t n, int64_t npf, int cpad, int nk,
-    int model_type, int cb_major) {
-  extern __shared__ float smem[];  // epilogue labels: waves x K x 64
-  if (st->stopped) return;
-  const int ncand = st->ncand;
-  const int W = blockDim.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int wave = uniform(threadIdx.x >> 6);
-  // Every wave is an independent (query, 64-candidate block) pair.  Blocks are
-  // dealt to the 8 XCDs round-robin; XCD x takes the x-th contiguous slice of
-  // the queries.  cb_major: all queries of a slice for candidate block 0,
-  // then block 1, ... -- concurrent waves gather from one 64-column slice of
-  // Xc (measured 1.6x faster on the n=65536 forward selection).  Otherwise the
-  // nblk blocks of one query are consecutive waves that read the query's D
-  // (and M) row once into L2.
-  const int nblk = cpad / 64;
-  const int64_t per_xcd = (n + 7) / 8;
-  const int64_t local = (int64_t)(blockIdx.x / 8) * W + wave;
-  const int64_t qi = cb_major ? local % per_xcd : local / nblk;
-  const int cb = (int)(cb_major ? local / per_xcd : local % nblk);
-  const int64_t pos = (int64_t)(blockIdx.x % 8) * per_xcd + qi;
-  if (pos >= n || cb >= nblk || qi >= per_xcd) return;  // whole wave; no block barriers
+__device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* lab, int64_t q,
+                                            int cb) {
+  if (a.st->stopped) return;
+  const int ncand = a.st->ncand;
   if (cb * 64 >= ncand) return;
+  const int64_t n = a.n, npf = a.npf;
+  const int cpad = a.cpad;
+  const int lane = threadIdx.x & 63;
   const int c = cb * 64 + lane;
   const bool live = c < ncand;
-  const int64_t q = pos;
 
-  const float* __restrict__ Dq = D + q * n;
-  const float* __restrict__ Mq = MULT > 0 ? nullptr : M + q * n;
-  const float* __restrict__ Xl = Xc + cb * 64 + lane;
+  const float* __restrict__ Dq = a.D + q * n;
+  const float* __restrict__ Mq = MULT > 0 ? nullptr : a.M + q * n;
+  const float* __restrict__ Xl = a.Xc + cb * 64 + lane;
   const float xq = Xl[q * cpad];
   const int64_t flo = (q / npf) * npf;
   const int64_t fhi = flo + npf < n ? flo + npf : n;
@@ -325,31 +261,29 @@ __global__ __launch_bounds__(256) void k_search_pruned(
   float B[K];
 #pragma unroll
   for (int m = 0; m < K; ++m) B[m] = kInf;
-  {
-    const int* __restrict__ sq = seeds + q * 128;
-    for (int s0 = 0; s0 < 128; s0 += 8) {
-      int js[8];
-      float xs[8], dqs[8];
+  const int* __restrict__ sq = a.seeds + q * 128;
+  for (int s0 = 0; s0 < 128; s0 += 8) {
+    int js[8];
+    float xs[8], dqs[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        js[u] = sq[s0 + u];
-        const int jj = js[u] < 0 ? 0 : js[u];
-        xs[u] = Xl[(int64_t)jj * cpad];
-        dqs[u] = Dq[jj];
-      }
+    for (int u = 0; u < 8; ++u) {
+      js[u] = sq[s0 + u];
+      const int jj = js[u] < 0 ? 0 : js[u];
+      xs[u] = Xl[(int64_t)jj * cpad];
+      dqs[u] = Dq[jj];
+    }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (js[u] >= 0) {
-          const float t = xq - xs[u];
-          const float p = t * t;
-          const float dd = dist(dqs[u], p);
-          if (dd < B[K - 1]) kth_insert<K>(B, dd);
-        }
+    for (int u = 0; u < 8; ++u) {
+      if (js[u] >= 0) {
+        const float t = xq - xs[u];
+        const float p = t * t;
+        const float dd = dist(dqs[u], p);
+        if (dd < B[K - 1]) kth_insert<K>(B, dd);
       }
     }
   }
   float thr = live ? B[K - 1] : -kInf;
-  float Tw = wave_max(thr);
+  float T = wave_max(thr);
 
   float Ld[K];
   int Li[K];
@@ -383,7 +317,7 @@ __global__ __launch_bounds__(256) void k_search_pruned(
       m2 = m3;
       m3 = ldM(jl + 256);
     }
-    const bool ok = jl < n && (jl < flo || jl >= fhi) && lb <= Tw;
+    const bool ok = jl < n && (jl < flo || jl >= fhi) && lb <= T;
     unsigned long long mask = __ballot(ok);
     if (!mask) continue;
     seen += __popcll(mask);
@@ -432,67 +366,211 @@ __global__ __launch_bounds__(256) void k_search_pruned(
     }
     if (__ballot(changed)) {
       thr = live ? fminf(thr, Ld[K - 1]) : -kInf;
-      Tw = wave_max(thr);
+      T = wave_max(thr);
     }
   }
-  if (visited && lane == 0) {
+  if (a.visited && lane == 0) {
     const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
-    atomicAdd(visited, seen * (unsigned long long)nlive);  // point evaluations performed
+    atomicAdd(a.visited, seen * (unsigned long long)nlive);  // point evaluations performed
   }
-  float* lab = smem + wave * (K * 64) + lane;
-  finish_query<K>(Li, lab, y, kvals, nk, model_type, P, q, c, cpad, n, live);
+  finish_query<K>(a, Li, lab, q, c, live);
+}
+
+// ----------------------------------------------------- single-problem kernels
+
+template <int K, int QW, int WAVES, int MULT, bool ZERO>
+__global__ __launch_bounds__(64 * WAVES) void k_search_dense(const SearchProblem a) {
+  constexpr int LDS_SCAN = 2 * 64 * 64;
+  constexpr int LDS_EPI = WAVES * K * 64;
+  __shared__ float lds[LDS_SCAN > LDS_EPI ? LDS_SCAN : LDS_EPI];
+  dense_block<K, QW, WAVES, MULT, ZERO>(a, lds, blockIdx.x, blockIdx.y);
+}
+
+// Every wave is an independent (query, 64-candidate block) item.  Blocks are
+// dealt to the 8 XCDs round-robin; XCD x takes the x-th contiguous slice of
+// the queries.  cb_major: all queries of a slice for candidate block 0, then
+// block 1, ... -- concurrent waves gather from one 64-column slice of Xc
+// (measured 1.6x faster on the n=65536 forward selection).  Otherwise the
+// nblk blocks of one query are consecutive waves that read the query's D (and
+// M) row once into L2.
+template <int K, int MULT>
+__global__ __launch_bounds__(256) void k_search_pruned(const SearchProblem a, int cb_major) {
+  __shared__ float lds[4 * K * 64];  // epilogue labels
+  const int wave = uniform(threadIdx.x >> 6);
+  const int nblk = a.cpad / 64;
+  const int64_t per_xcd = (a.n + 7) / 8;
+  const int64_t local = (int64_t)(blockIdx.x / 8) * 4 + wave;
+  const int64_t qi = cb_major ? local % per_xcd : local / nblk;
+  const int cb = (int)(cb_major ? local / per_xcd : local % nblk);
+  const int64_t q = (int64_t)(blockIdx.x % 8) * per_xcd + qi;
+  if (q >= a.n || cb >= nblk || qi >= per_xcd) return;  // whole wave; no barriers
+  pruned_wave<K, MULT>(a, lds + wave * (K * 64) + (threadIdx.x & 63), q, cb);
+}
+
+// ------------------------------------------------------ persistent batch kernels
+//
+// Many independent problems in one launch: a global counter hands out items
+// (problem, query block or query, candidate block) in problem order; item
+// ranges per problem are in `start` (prefix sums, P+1 entries).  Each item
+// runs the body of the problem's list size K.  The counter is reset by a
+// kernel before every launch (no memset nodes in a captured graph).
+
+#define SNN_BATCH_SIZES(X) X(2) X(4) X(5) X(8) X(10) X(15) X(16) X(20) X(24)
+constexpr int kBatchMaxK = 24;
+constexpr int kBatchWaves = 8;  // dense batch block: 8 waves x 2 queries
+
+__device__ __forceinline__ int find_problem(const int64_t* start, int P, int64_t item) {
+  int lo = 0, hi = P;  // largest p with start[p] <= item
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Dense items of a problem: (candidate block, query block of 2*kBatchWaves
+// queries), candidate-block major.  All problems of one launch share MULT.
+template <int MULT, bool ZERO>
+__global__ __launch_bounds__(64 * kBatchWaves) void k_batch_dense(
+    const SearchProblem* __restrict__ probs, const int64_t* __restrict__ start, int P,
+    unsigned long long* __restrict__ counter) {
+  constexpr int LDS_SCAN = 2 * 64 * 64;
+  constexpr int LDS_EPI = kBatchWaves * kBatchMaxK * 64;
+  __shared__ float lds[LDS_SCAN > LDS_EPI ? LDS_SCAN : LDS_EPI];
+  __shared__ long long s_item;
+  const int64_t total = start[P];
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (long long)atomicAdd(counter, 1ull);
+    __syncthreads();
+    const int64_t item = uniform64(s_item);
+    __syncthreads();
+    if (item >= total) break;
+    const int p = find_problem(start, P, item);
+    const SearchProblem a = probs[p];
+    const int64_t local = item - start[p];
+    const int64_t qblocks = (a.n + 2 * kBatchWaves - 1) / (2 * kBatchWaves);
+    const int cb = (int)(local / qblocks);
+    const int64_t qb = local % qblocks;
+    switch (search_list_size(a.kmax)) {
+#define SNN_CASE(KK) \
+  case KK: dense_block<KK, 2, kBatchWaves, MULT, ZERO>(a, lds, qb, cb); break;
+      SNN_BATCH_SIZES(SNN_CASE)
+#undef SNN_CASE
+      default: break;
+    }
+  }
+}
+
+// Pruned items of a problem: (candidate block, query), candidate-block major;
+// one wave per item, no barriers.
+template <int MULT>
+__global__ __launch_bounds__(256) void k_batch_pruned(const SearchProblem* __restrict__ probs,
+                                                      const int64_t* __restrict__ start, int P,
+                                                      unsigned long long* __restrict__ counter) {
+  __shared__ float lds[4 * kBatchMaxK * 64];
+  const int wave = uniform(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  float* lab = lds + wave * (kBatchMaxK * 64) + lane;
+  const int64_t total = start[P];
+  for (;;) {
+    unsigned long long it = 0;
+    if (lane == 0) it = atomicAdd(counter, 1ull);
+    const int64_t item = uniform64(__shfl((long long)it, 0));
+    if (item >= total) break;
+    const int p = find_problem(start, P, item);
+    const SearchProblem a = probs[p];
+    const int64_t local = item - start[p];
+    const int cb = (int)(local / a.n);
+    const int64_t q = local % a.n;
+    switch (search_list_size(a.kmax)) {
+#define SNN_CASE(KK) \
+  case KK: pruned_wave<KK, MULT>(a, lab, q, cb); break;
+      SNN_BATCH_SIZES(SNN_CASE)
+#undef SNN_CASE
+      default: break;
+    }
+  }
+}
+
+__global__ void k_reset_counter(unsigned long long* c) { *c = 0ull; }
+
+// Seeds of the pruning bound: for each query, lane L of one wave keeps the two
+// smallest eligible D_qj over the residue class j = L (mod 64), ties to the
+// lower index; 128 distinct points, -1 where a class has fewer than two.
+__global__ __launch_bounds__(256) void k_seed_scan(const float* __restrict__ D,
+                                                   const SelState* __restrict__ st,
+                                                   int* __restrict__ seeds, int64_t n,
+                                                   int64_t npf) {
+  if (st->stopped) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t q = (int64_t)blockIdx.x * 4 + uniform(threadIdx.x >> 6);
+  if (q >= n) return;
+  const float* __restrict__ Dq = D + q * n;
+  const int64_t flo = (q / npf) * npf;
+  const int64_t fhi = flo + npf < n ? flo + npf : n;
+  const float kInf = __int_as_float(0x7f800000);
+  float b0 = kInf, b1 = kInf;
+  int i0 = -1, i1 = -1;
+  for (int64_t j = lane; j < n; j += 64) {
+    if (j >= flo && j < fhi) continue;
+    const float v = Dq[j];
+    if (v < b1) {
+      if (v < b0) {
+        b1 = b0;
+        i1 = i0;
+        b0 = v;
+        i0 = (int)j;
+      } else {
+        b1 = v;
+        i1 = (int)j;
+      }
+    }
+  }
+  seeds[q * 128 + lane] = i0;
+  seeds[q * 128 + 64 + lane] = i1;
 }
 
 // ============================================================ launchers
 
 template <int K, int QW, int WAVES>
-static hipError_t launch_dense_k(const SearchLaunch& a, hipStream_t s) {
+static hipError_t launch_dense_k(const SearchProblem& a, hipStream_t s) {
   const int qpb = QW * WAVES;
   dim3 g((unsigned)((a.n + qpb - 1) / qpb), (unsigned)(a.cpad / 64));
   dim3 b(64 * WAVES);
   if (a.mult < 0)
-    hipLaunchKernelGGL((k_search_dense<K, QW, WAVES, -1, false>), g, b, 0, s, a.D, a.Xc, a.y,
-                       a.kvals, a.st, a.P, a.n, a.npf, a.cpad, a.nk, a.model_type);
+    hipLaunchKernelGGL((k_search_dense<K, QW, WAVES, -1, false>), g, b, 0, s, a);
   else if (a.zero)
-    hipLaunchKernelGGL((k_search_dense<K, QW, WAVES, 1, true>), g, b, 0, s, a.D, a.Xc, a.y,
-                       a.kvals, a.st, a.P, a.n, a.npf, a.cpad, a.nk, a.model_type);
+    hipLaunchKernelGGL((k_search_dense<K, QW, WAVES, 1, true>), g, b, 0, s, a);
   else
-    hipLaunchKernelGGL((k_search_dense<K, QW, WAVES, 1, false>), g, b, 0, s, a.D, a.Xc, a.y,
-                       a.kvals, a.st, a.P, a.n, a.npf, a.cpad, a.nk, a.model_type);
+    hipLaunchKernelGGL((k_search_dense<K, QW, WAVES, 1, false>), g, b, 0, s, a);
   return hipGetLastError();
 }
 
 template <int K>
-static hipError_t launch_pruned_k(const SearchLaunch& a, hipStream_t s) {
-  // 4 independent waves per block; each XCD slice holds per_xcd*nblk waves
-  const int waves = 4;
-  const int64_t per_xcd = (a.n + 7) / 8;
-  const int64_t slice_blocks = (per_xcd * (a.cpad / 64) + waves - 1) / waves;
-  const size_t lds = sizeof(float) * (size_t)waves * K * 64;
-  const dim3 g((unsigned)(8 * slice_blocks)), b(64 * waves);
+static hipError_t launch_pruned_k(const SearchProblem& a, hipStream_t s) {
+  // 4 independent waves per block; each XCD slice holds per_xcd*nblk waves.
   // forward: gather locality (candidate-block major); backward: the D and M
   // rows dominate (query major)
-  if (a.mult > 0)
-    hipLaunchKernelGGL((k_search_pruned<K, 1>), g, b, lds, s, a.D, a.M, a.Xc, a.y, a.kvals, a.st,
-                       a.seeds, a.P, a.visited, a.n, a.npf, a.cpad, a.nk, a.model_type, 1);
-  else
-    hipLaunchKernelGGL((k_search_pruned<K, -1>), g, b, lds, s, a.D, a.M, a.Xc, a.y, a.kvals, a.st,
-                       a.seeds, a.P, a.visited, a.n, a.npf, a.cpad, a.nk, a.model_type, 0);
+  const int64_t per_xcd = (a.n + 7) / 8;
+  const int64_t slice_blocks = (per_xcd * (a.cpad / 64) + 3) / 4;
+  const dim3 g((unsigned)(8 * slice_blocks)), b(256);
+  if (a.mult > 0) hipLaunchKernelGGL((k_search_pruned<K, 1>), g, b, 0, s, a, 1);
+  else hipLaunchKernelGGL((k_search_pruned<K, -1>), g, b, 0, s, a, 0);
   return hipGetLastError();
-}
-
-int search_list_size(int kmax) {
-  static const int sizes[] = {2, 4, 5, 8, 10, 15, 16, 20, 24, 32, 50, 64};
-  for (int v : sizes)
-    if (kmax <= v) return v;
-  return -1;
 }
 
 #define SNN_FOR_EACH_LIST_SIZE(X) \
   X(2, 2, 8) X(4, 2, 8) X(5, 2, 8) X(8, 2, 8) X(10, 2, 8) X(15, 2, 8) X(16, 2, 8) X(20, 2, 8) \
   X(24, 2, 8) X(32, 1, 8) X(50, 1, 4) X(64, 1, 4)
 
-hipError_t launch_search_dense(const SearchLaunch& a, hipStream_t s) {
+hipError_t launch_search_dense(const SearchProblem& a, hipStream_t s) {
   switch (search_list_size(a.kmax)) {
 #define SNN_CASE(K, QW, W) \
   case K: return launch_dense_k<K, QW, W>(a, s);
@@ -502,7 +580,7 @@ hipError_t launch_search_dense(const SearchLaunch& a, hipStream_t s) {
   }
 }
 
-hipError_t launch_search_pruned(const SearchLaunch& a, hipStream_t s) {
+hipError_t launch_search_pruned(const SearchProblem& a, hipStream_t s) {
   switch (search_list_size(a.kmax)) {
 #define SNN_CASE(K, QW, W) \
   case K: return launch_pruned_k<K>(a, s);
@@ -510,6 +588,52 @@ hipError_t launch_search_pruned(const SearchLaunch& a, hipStream_t s) {
 #undef SNN_CASE
     default: return hipErrorInvalidValue;
   }
+}
+
+int batch_max_list_size() { return kBatchMaxK; }
+
+int64_t batch_dense_items(const SearchProblem& a) {
+  return ((a.n + 2 * kBatchWaves - 1) / (2 * kBatchWaves)) * (a.cpad / 64);
+}
+
+int64_t batch_pruned_items(const SearchProblem& a) { return a.n * (a.cpad / 64); }
+
+int batch_grid(int mode) {
+  // resident blocks per CU x CUs: the persistent grid fills the chip once
+  static int cached[2] = {0, 0};
+  int& g = cached[mode == 2 ? 1 : 0];
+  if (g) return g;
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 256;
+  hipError_t e;
+  if (mode == 2)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_pruned<1>, 256, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_dense<1, false>,
+                                                     64 * kBatchWaves, 0);
+  if (e != hipSuccess || per < 1) per = 1;
+  g = cus * per;
+  return g;
+}
+
+hipError_t launch_batch_search(const SearchProblem* probs, const int64_t* start, int P,
+                               unsigned long long* counter, int mode, int mult, int grid,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, counter);
+  const dim3 gp(grid), bp(256), bd(64 * kBatchWaves);
+  if (mode == 2) {
+    if (mult > 0) hipLaunchKernelGGL(k_batch_pruned<1>, gp, bp, 0, s, probs, start, P, counter);
+    else hipLaunchKernelGGL(k_batch_pruned<-1>, gp, bp, 0, s, probs, start, P, counter);
+  } else if (mult < 0) {
+    hipLaunchKernelGGL((k_batch_dense<-1, false>), gp, bd, 0, s, probs, start, P, counter);
+  } else if (mode == 1) {
+    hipLaunchKernelGGL((k_batch_dense<1, true>), gp, bd, 0, s, probs, start, P, counter);
+  } else {
+    hipLaunchKernelGGL((k_batch_dense<1, false>), gp, bd, 0, s, probs, start, P, counter);
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_seed_scan(const float* D, const SelState* st, int* seeds, int64_t n,

@@ -142,3 +142,20 @@ def test_workload_shapes():
     w = ds.config("c0")
     assert w["X"].shape == (2000, 50) and set(np.unique(w["y"])) == {-1.0, 1.0}
     assert int((w["y"] == 1).sum()) + int((w["y"] == -1).sum()) == 2000
+
+
+def test_reference_qsort_order_matches_the_c_library(tmp_path):
+    """The device's median sort for NaN-holding arrays (qsort_order.h) leaves
+    every array exactly as the C library's qsort does with the reference's
+    comparator, NaN, infinities, signed zeros and ties included; so does its
+    64-worker split used by one wave."""
+    import shutil
+    import subprocess
+    if not shutil.which("g++"):
+        pytest.skip("g++ not available")
+    exe = tmp_path / "qsort_order_check"
+    src = os.path.join(ROOT, "tests", "qsort_order_check.cpp")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-o", str(exe), src],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.startswith("ok"), out.stdout
