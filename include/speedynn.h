@@ -142,7 +142,7 @@ typedef struct {
     int64_t n;
     int d;
     const int *k_vals;
-    int num_k;                   /* largest k value <= 24 in a batch */
+    int num_k;
     const int *loss_functions;
     int num_loss;
     snn_params params;           /* form and use_graph must agree across the batch */
@@ -156,11 +156,12 @@ typedef struct {
 } snn_problem;
 
 /* Many independent selections (uneven n, d and k) advanced together: every
- * step runs the candidate search of all problems in one persistent launch
- * (dense for a forward selection's first step, pruned afterwards in the
- * precision form), with the per-problem gather, losses, arg-min and update
- * around it; with params.use_graph the step is captured once in a hipGraph
- * and replayed.  Problems whose arguments are invalid get their status set and
+ * step runs the candidate search of all problems with the same direction and
+ * neighbour-list length in one persistent launch over a shared work queue
+ * (dense in the plain form; in the precision form a forward selection's first
+ * step runs over sorted columns and later steps run the pruned search), with
+ * the per-problem gather, losses, arg-min and update around it; with
+ * params.use_graph the step is captured once in a hipGraph and replayed.  Problems whose arguments are invalid get their status set and
  * are left out; the call returns SNN_EINVAL if any was, after running the
  * others.  No counterpart in the reference, which selects one problem per call
  * (speedynn/lib/nearest_neighbors_feature_selection.c:49-65).  summary

@@ -32,8 +32,10 @@ struct Slot {
   Problem pb;
   int64_t work;
   size_t X, y, XT, D, Dlo, M, Xc, P, err, val, bestk, cand, sel, ordering, errors, kvals, losses,
-      state, perm, key, seeds, med;
-  bool has_med;  // median losses: sort space for NaN-holding error vectors
+      state, perm, key, seeds, med, skey, sidx, sort_tmp;
+  bool has_med;      // median losses: sort space for NaN-holding error vectors
+  bool sorted;       // forward, precision form: sorted columns for the first step
+  size_t sort_bytes;
 };
 
 size_t take(size_t& off, size_t bytes) {
@@ -47,11 +49,14 @@ T* at(const DevBuf& b, size_t off) {
   return reinterpret_cast<T*>(static_cast<char*>(b.p) + off);
 }
 
-// Problems of one direction: a contiguous range of the device problem array
-// (largest work first, so the queue hands out the long items early).
+// Problems of one direction and one list size: a contiguous range of the
+// device problem array (largest work first, so the queue hands out the long
+// items early) served by one persistent launch.
 struct Group {
   int first = 0, count = 0;
   int mult = 1;
+  int K = 0;
+  int grid_dense = 0, grid_pruned = 0;
   size_t dense_start = 0, pruned_start = 0;  // prefix arrays in meta
 };
 
@@ -79,15 +84,17 @@ struct Plan {
   std::vector<Slot> slots;  // device order: groups contiguous
   std::vector<Group> groups;
   bool pruned;
+  bool dense_first;  // some forward problem holds non-finite X: dense first step for it
   size_t probs_off, counter_off, visited_off;
-  int grid_dense, grid_pruned;
+  std::vector<snn::SearchProblem> hp;  // host copies of the device descriptors
 
   const DevBuf& A() const { return ctx->batch.arena; }
   const DevBuf& Mt() const { return ctx->batch.meta; }
 
   // Launches of one selection step (step > 0 differs only for a forward
-  // selection in the precision form: its first step runs the dense search
-  // with D = 0, later ones the pruned search).
+  // selection in the precision form: its first step, D = 0, runs over sorted
+  // columns -- one launch per problem, a short step -- later ones run the
+  // pruned search).
   int issue_step(bool first_step, hipStream_t s, hipEvent_t ev_search0,
                  hipEvent_t ev_search1) const {
     for (const Slot& sl : slots) {
@@ -109,11 +116,16 @@ struct Plan {
                                        at<int>(A(), sl.seeds), sl.pb.n, sl.pb.npf, s));
         }
         HIPCHK(snn::launch_batch_search(probs + g.first, at<int64_t>(Mt(), g.pruned_start),
-                                        g.count, counter, 2, g.mult, grid_pruned, s));
+                                        g.count, counter, 2, g.mult, g.K, g.grid_pruned, s));
+      } else if (pruned && g.mult > 0) {
+        for (int i = g.first; i < g.first + g.count; ++i)
+          HIPCHK(snn::launch_search_sorted(hp[i], slots[i].pb.d, s));
+        if (dense_first)
+          HIPCHK(snn::launch_batch_search(probs + g.first, at<int64_t>(Mt(), g.dense_start),
+                                          g.count, counter, 1, g.mult, g.K, g.grid_dense, s));
       } else {
-        const int mode = (pruned && g.mult > 0) ? 1 : 0;
         HIPCHK(snn::launch_batch_search(probs + g.first, at<int64_t>(Mt(), g.dense_start),
-                                        g.count, counter, mode, g.mult, grid_dense, s));
+                                        g.count, counter, 0, g.mult, g.K, g.grid_dense, s));
       }
     }
     if (ev_search1) HIPCHK(hipEventRecord(ev_search1, s));
@@ -171,9 +183,6 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     int rc = (q.X && q.y) ? snn::validate_problem(q.n, q.d, q.k_vals, q.num_k, q.loss_functions,
                                                   q.num_loss, &q.params, &pb)
                           : fail(SNN_EINVAL, "null input");
-    if (!rc && snn::search_list_size(pb.kmax) > snn::batch_max_list_size())
-      rc = fail(SNN_ENOTSUP, "k values above " + std::to_string(snn::batch_max_list_size()) +
-                                 " are not supported in a batch");
     if (!rc && form >= 0 &&
         (q.params.form != form || (q.params.use_graph != 0) != (use_graph != 0)))
       rc = fail(SNN_EINVAL, "form and use_graph must agree across the batch");
@@ -198,20 +207,28 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
   plan.ctx = ctx;
   plan.pruned = form == SNN_FORM_PRECISION;
   const bool pruned = plan.pruned;
+  HIPCHK(hipSetDevice(ctx->device));
 
-  // ---- device order: forward group, then backward group, each by work descending
+  // ---- device order: groups of one direction and list size, each by work descending
   for (int dir = 1; dir >= 0; --dir) {
-    Group g;
-    g.first = (int)plan.slots.size();
-    g.mult = dir == 1 ? 1 : -1;
-    std::vector<Slot> part;
-    for (const Slot& s : valid)
-      if (s.pb.method == dir) part.push_back(s);
-    std::stable_sort(part.begin(), part.end(),
-                     [](const Slot& a, const Slot& b) { return a.work > b.work; });
-    for (const Slot& s : part) plan.slots.push_back(s);
-    g.count = (int)part.size();
-    plan.groups.push_back(g);
+    for (int K = 1; K <= 64; ++K) {
+      if (snn::search_list_size(K) != K) continue;  // compiled list sizes only
+      Group g;
+      g.first = (int)plan.slots.size();
+      g.mult = dir == 1 ? 1 : -1;
+      g.K = K;
+      std::vector<Slot> part;
+      for (const Slot& s : valid)
+        if (s.pb.method == dir && snn::search_list_size(s.pb.kmax) == K) part.push_back(s);
+      if (part.empty()) continue;
+      std::stable_sort(part.begin(), part.end(),
+                       [](const Slot& a, const Slot& b) { return a.work > b.work; });
+      for (const Slot& s : part) plan.slots.push_back(s);
+      g.count = (int)part.size();
+      g.grid_dense = snn::batch_grid(pruned && dir == 1 ? 1 : 0, g.mult, K);
+      g.grid_pruned = snn::batch_grid(2, g.mult, K);
+      plan.groups.push_back(g);
+    }
   }
 
   // ---- arena layout
@@ -243,6 +260,12 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     const snn_problem& q = probs[s.index];
     s.has_med = snn::needs_median_scratch(q.loss_functions, q.num_loss);
     s.med = s.has_med ? take(off, sizeof(float) * 2 * n * cpad * pb.nk) : 0;
+    s.sorted = pruned && pb.method == 1;
+    s.sort_bytes = s.sorted ? snn::sort_columns_temp_bytes(pb.n, pb.d) : 0;
+    if (s.sorted && !s.sort_bytes) return fail(SNN_EDEVICE, "segmented sort: no temporary size");
+    s.skey = s.sorted ? take(off, sizeof(float) * n * d) : 0;
+    s.sidx = s.sorted ? take(off, sizeof(int) * n * d) : 0;
+    s.sort_tmp = s.sorted ? take(off, s.sort_bytes) : 0;
   }
   const size_t arena_bytes = off;
   const int B = (int)plan.slots.size();
@@ -264,12 +287,17 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
   hipStream_t s = ctx->stream;
 
   // ---- host copies of the search descriptors and work queues
-  std::vector<snn::SearchProblem> hp(B);
+  std::vector<snn::SearchProblem>& hp = plan.hp;
+  hp.assign(B, snn::SearchProblem{});
   std::vector<char> hmeta(moff, 0);
   for (int i = 0; i < B; ++i) {
     const Slot& sl = plan.slots[i];
     const Problem& pb = sl.pb;
     snn::SearchProblem& a = hp[i];
+    a.sorted = sl.sorted ? 1 : 0;
+    a.skey = sl.sorted ? at<float>(cache.arena, sl.skey) : nullptr;
+    a.sidx = sl.sorted ? at<int>(cache.arena, sl.sidx) : nullptr;
+    a.cand = at<int>(cache.arena, sl.cand);
     a.D = at<float>(cache.arena, sl.D);
     a.M = (pruned && pb.method == 0) ? at<float>(cache.arena, sl.M) : nullptr;
     a.Xc = at<float>(cache.arena, sl.Xc);
@@ -299,8 +327,6 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     }
   }
   HIPCHK(hipMemcpyAsync(cache.meta.p, hmeta.data(), moff, hipMemcpyHostToDevice, s));
-  plan.grid_dense = snn::batch_grid(0);
-  plan.grid_pruned = snn::batch_grid(2);
 
   // ---- inputs and per-problem start state
   Timing tm;
@@ -333,6 +359,11 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
                                   at<int>(cache.arena, sl.ordering),
                                   at<float>(cache.arena, sl.errors), at<int>(cache.arena, sl.perm),
                                   s));
+    if (sl.sorted)
+      HIPCHK(snn::launch_sort_columns(XT, pb.n, pb.d, at<SelState>(cache.arena, sl.state),
+                                      at<float>(cache.arena, sl.skey),
+                                      at<int>(cache.arena, sl.sidx),
+                                      at<void>(cache.arena, sl.sort_tmp), sl.sort_bytes, s));
     if (pb.method == 1) {
       HIPCHK(snn::launch_fill(D, nn, 0.0f, s));
       if (Dlo) HIPCHK(snn::launch_fill(Dlo, nn, 0.0f, s));
@@ -342,6 +373,17 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     }
   }
   HIPCHK(hipEventRecord(e1, s));
+  // does any sorted problem hold non-finite X (then the dense first step runs
+  // for it; the sorted and dense kernels each check the problem's flag)?
+  plan.dense_first = false;
+  for (const Slot& sl : plan.slots) {
+    if (!sl.sorted) continue;
+    SelState hs{};
+    HIPCHK(hipMemcpyAsync(&hs, at<SelState>(cache.arena, sl.state), sizeof(hs),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    plan.dense_first |= hs.nonfinite != 0;
+  }
 
   // ---- steps
   int max_steps = 0;
@@ -349,7 +391,7 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
   double ms_search = 0.0;
   if (use_graph) {
     std::vector<size_t> key{(size_t)cache.arena.p, (size_t)cache.meta.p, (size_t)form,
-                            (size_t)B};
+                            (size_t)B, (size_t)plan.dense_first};
     for (const Slot& sl : plan.slots) {
       const Problem& pb = sl.pb;
       const int64_t f[] = {pb.n, pb.npf, pb.d, pb.nk, pb.nl, pb.kmax, pb.method, pb.model_type,

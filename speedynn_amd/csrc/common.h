@@ -28,7 +28,8 @@ struct SelState {
   int pos;        // record position of the next step
   int pending;    // fstar chosen but not yet applied to D
   int fslot;      // candidate slot of fstar in the step that chose it
-  int pad[2];
+  int nonfinite;  // X holds NaN or infinity (the sorted first step does not apply)
+  int pad;
 };
 
 // Fixed description of one selection problem on the device.
@@ -67,6 +68,10 @@ struct SearchProblem {
   int mult;              // +1 forward, -1 backward
   int model_type;
   int zero;              // D is known to be 0 (first forward step)
+  int sorted;            // first forward step: sorted columns serve finite X
+  const float* skey;     // [d][n] each feature's values, ascending
+  const int* sidx;       // [d][n] their point indices
+  const int* cand;       // [d] feature of each candidate slot
 };
 
 // Neighbour-list length compiled for a largest k value (-1: above 64).

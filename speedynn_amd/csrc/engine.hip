@@ -155,6 +155,16 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     HIPCHK(ctx->visited.ensure(sizeof(unsigned long long)));
     HIPCHK(snn::launch_fill_u64(ctx->visited.as<unsigned long long>(), 1, ctx->stream));
   }
+  // precision form, forward: the first step runs over sorted feature columns
+  const bool sorted_first = pruned && fwd;
+  size_t sort_bytes = 0;
+  if (sorted_first) {
+    HIPCHK(ctx->skey.ensure(sizeof(float) * (size_t)n * d));
+    HIPCHK(ctx->sidx.ensure(sizeof(int) * (size_t)n * d));
+    sort_bytes = snn::sort_columns_temp_bytes(n, d);
+    if (!sort_bytes) return fail(SNN_EDEVICE, "segmented sort: no temporary storage size");
+    HIPCHK(ctx->sort_tmp.ensure(sort_bytes));
+  }
   float* med = nullptr;  // median losses: sort space for NaN-holding error vectors
   if (snn::needs_median_scratch(loss_functions, num_loss)) {
     HIPCHK(ctx->med.ensure(sizeof(float) * 2 * (size_t)n * pb.cpad * num_k));
@@ -187,6 +197,9 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   HIPCHK(snn::launch_init_state(st, pb, ctx->cand.as<int>(), ctx->sel.as<int>(),
                                 ctx->ordering.as<int>(), ctx->errors.as<float>(),
                                 ctx->perm.as<int>(), s));
+  if (sorted_first)
+    HIPCHK(snn::launch_sort_columns(ctx->XT.as<float>(), n, d, st, ctx->skey.as<float>(),
+                                    ctx->sidx.as<int>(), ctx->sort_tmp.p, sort_bytes, s));
   if (fwd) {
     HIPCHK(snn::launch_fill(D, (int64_t)nn, 0.0f, s));
     if (Dlo) HIPCHK(snn::launch_fill(Dlo, (int64_t)nn, 0.0f, s));
@@ -211,6 +224,12 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   sl.mult = fwd ? 1 : -1;
   sl.model_type = prm->model_type;
   sl.zero = 0;
+  if (sorted_first) {
+    sl.sorted = 1;
+    sl.skey = ctx->skey.as<float>();
+    sl.sidx = ctx->sidx.as<int>();
+    sl.cand = ctx->cand.as<int>();
+  }
   if (pruned) {
     sl.M = fwd ? nullptr : ctx->M.as<float>();
     sl.seeds = ctx->seeds.as<int>();
@@ -228,7 +247,10 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
       HIPCHK(snn::launch_seed_scan(D, st, ctx->seeds.as<int>(), n, pb.npf, s));
       HIPCHK(snn::launch_search_pruned(sl, s));
     } else {
-      sl.zero = pruned ? 1 : 0;  // first forward step: D is 0
+      // first forward step (D = 0): sorted columns; the dense search runs
+      // instead where X is not finite (each kernel checks the state's flag)
+      if (sorted_first) HIPCHK(snn::launch_search_sorted(sl, d, s));
+      sl.zero = pruned ? 1 : 0;
       HIPCHK(snn::launch_search_dense(sl, s));
     }
     HIPCHK(hipEventRecord(c, s));
@@ -374,7 +396,8 @@ void snn_destroy(snn_ctx* c) {
   DevBuf* bufs[] = {&c->X, &c->y, &c->XT, &c->D, &c->Dlo, &c->Xc, &c->P, &c->err, &c->val,
                     &c->bestk, &c->cand, &c->sel, &c->ordering, &c->errors, &c->cand_err,
                     &c->kvals, &c->losses, &c->state, &c->scratch, &c->seeds,
-                    &c->visited, &c->perm, &c->key, &c->M, &c->med};
+                    &c->visited, &c->perm, &c->key, &c->M, &c->med, &c->skey,
+                    &c->sidx, &c->sort_tmp};
   for (DevBuf* b : bufs) b->release();
   c->batch.drop_graphs();
   c->batch.arena.release();

@@ -15,6 +15,15 @@ hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* 
                           int d, int cpad, hipStream_t s);
 hipError_t launch_search_dense(const SearchProblem& a, hipStream_t s);
 hipError_t launch_search_pruned(const SearchProblem& a, hipStream_t s);
+// First forward step over sorted columns (a.skey, a.sidx, a.cand); grid over
+// max_cand candidate slots.  Returns at once where st->nonfinite is set.
+hipError_t launch_search_sorted(const SearchProblem& a, int max_cand, hipStream_t s);
+// Sorted columns of XT: skey[f][.] ascending, sidx[f][.] their point indices
+// (stable: equal values keep ascending index); also sets st->nonfinite when X
+// holds NaN or infinity.  temp: sort_columns_temp_bytes(n, d) bytes.
+size_t sort_columns_temp_bytes(int64_t n, int d);
+hipError_t launch_sort_columns(const float* XT, int64_t n, int d, SelState* st, float* skey,
+                               int* sidx, void* temp, size_t temp_bytes, hipStream_t s);
 hipError_t launch_seed_scan(const float* D, const SelState* st, int* seeds, int64_t n,
                             int64_t npf, hipStream_t s);
 // scratch: median losses with NaN errors sort a copy there, 2n floats per
@@ -37,13 +46,13 @@ hipError_t launch_init_state(SelState* st, const Problem& pb, int* cand, int* se
 // Persistent search over many problems (device array `probs`, item prefix
 // sums `start[P+1]`, work counter).  mode 0: dense, 1: dense with D = 0
 // (first forward step), 2: pruned; every problem of a launch has direction
-// `mult` (+1 forward, -1 backward).  List sizes up to batch_max_list_size().
-int batch_max_list_size();
+// `mult` (+1 forward, -1 backward) and list size K.  batch_grid: resident
+// blocks of that kernel on the current device (call before graph capture).
 int64_t batch_dense_items(const SearchProblem& a);
 int64_t batch_pruned_items(const SearchProblem& a);
-int batch_grid(int mode);  // persistent grid size for a mode on the current device
+int batch_grid(int mode, int mult, int K);
 hipError_t launch_batch_search(const SearchProblem* probs, const int64_t* start, int P,
-                               unsigned long long* counter, int mode, int mult, int grid,
+                               unsigned long long* counter, int mode, int mult, int K, int grid,
                                hipStream_t s);
 
 }  // namespace snn

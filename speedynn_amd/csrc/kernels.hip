@@ -582,6 +582,8 @@ __global__ void k_init_state(SelState* __restrict__ st, const Problem pb, int* _
   if (t == 0) {
     st->step = 0;
     st->fstar = -1;
+    st->nonfinite = 0;
+    st->fslot = 0;
     st->pending = 0;
     st->best_k = 0;
     st->error = 0;

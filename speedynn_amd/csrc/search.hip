@@ -91,6 +91,7 @@ __device__ __forceinline__ void dense_block(const SearchProblem& a, float* lds, 
                                             int cb) {
   constexpr int J = 64;
   if (a.st->stopped) return;
+  if (ZERO && a.sorted && !a.st->nonfinite) return;  // k_search_sorted serves this step
   const int ncand = a.st->ncand;
   if (cb * 64 >= ncand) return;
   const int64_t n = a.n, npf = a.npf;
@@ -376,6 +377,93 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* lab, 
   finish_query<K>(a, Li, lab, q, c, live);
 }
 
+// Top-K insertion ordered by (distance, index) whatever the offering order:
+// the list ends as the K smallest pairs, which is what the ascending-index
+// scan with strict comparisons produces.
+__device__ __forceinline__ bool pair_less(float a, int ja, float b, int jb) {
+  return a < b || (a == b && ja < jb);
+}
+
+template <int K>
+__device__ __forceinline__ void topk_insert_pair(float (&Ld)[K], int (&Li)[K], float dd, int j) {
+#pragma unroll
+  for (int m = K - 1; m > 0; --m) {
+    const bool cp = pair_less(dd, j, Ld[m - 1], Li[m - 1]);
+    const bool ch = pair_less(dd, j, Ld[m], Li[m]);
+    const float nd = cp ? Ld[m - 1] : (ch ? dd : Ld[m]);
+    const int ni = cp ? Li[m - 1] : (ch ? j : Li[m]);
+    Ld[m] = nd;
+    Li[m] = ni;
+  }
+  if (pair_less(dd, j, Ld[0], Li[0])) {
+    Ld[0] = dd;
+    Li[0] = j;
+  }
+}
+
+// First step of a forward selection (D = 0) over sorted columns: one lane per
+// (sorted position p of candidate f's column, f).  The query is the point at
+// p; walking outward, d = fl((x_q - x_j)^2) is non-decreasing on each side,
+// so the two sides are merged by distance and a side stops at the first
+// point whose d exceeds the list's K-th distance (points of equal distance
+// still enter when their index is lower).  Points of the query's fold are
+// passed over.  Consecutive lanes hold neighbouring positions, so their walks
+// read the same cache lines.  Needs finite X (st->nonfinite == 0).
+template <int K>
+__global__ __launch_bounds__(256) void k_search_sorted(const SearchProblem a) {
+  __shared__ float lds[4 * K * 64];
+  const SelState* st = a.st;
+  if (st->stopped || st->nonfinite) return;
+  const int c = blockIdx.y;
+  if (c >= st->ncand) return;
+  const int f = a.cand[c];
+  const int64_t n = a.n, npf = a.npf;
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool live = p < n;
+  const int64_t pp = live ? p : n - 1;
+  const float* __restrict__ key = a.skey + (int64_t)f * n;
+  const int* __restrict__ idx = a.sidx + (int64_t)f * n;
+  const float xq = key[pp];
+  const int q = idx[pp];
+  const int64_t flo = ((int64_t)q / npf) * npf, fhi = flo + npf;
+  const float kInf = __int_as_float(0x7f800000);
+
+  float Ld[K];
+  int Li[K];
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    Ld[m] = kMaxF;
+    Li[m] = 0;
+  }
+  auto dist = [&](int64_t t) {
+    const float u = xq - key[t];
+    return u * u;
+  };
+  int64_t l = pp - 1, r = pp + 1;
+  float dl = l >= 0 ? dist(l) : kInf, dr = r < n ? dist(r) : kInf;
+  int jl = l >= 0 ? idx[l] : 0, jr = r < n ? idx[r] : 0;
+  for (;;) {
+    const float lim = Ld[K - 1];
+    const bool lok = dl <= lim, rok = dr <= lim;
+    if (!lok && !rok) break;
+    const bool left = lok && (!rok || dl <= dr);
+    const float dd = left ? dl : dr;
+    const int j = left ? jl : jr;
+    if (j < flo || j >= fhi) topk_insert_pair<K>(Ld, Li, dd, j);
+    if (left) {
+      --l;
+      dl = l >= 0 ? dist(l) : kInf;
+      jl = l >= 0 ? idx[l] : 0;
+    } else {
+      ++r;
+      dr = r < n ? dist(r) : kInf;
+      jr = r < n ? idx[r] : 0;
+    }
+  }
+  const int wave = threadIdx.x >> 6;
+  finish_query<K>(a, Li, lds + wave * (K * 64) + (threadIdx.x & 63), q, c, live);
+}
+
 // ----------------------------------------------------- single-problem kernels
 
 template <int K, int QW, int WAVES, int MULT, bool ZERO>
@@ -409,15 +497,14 @@ __global__ __launch_bounds__(256) void k_search_pruned(const SearchProblem a, in
 
 // ------------------------------------------------------ persistent batch kernels
 //
-// Many independent problems in one launch: a global counter hands out items
-// (problem, query block or query, candidate block) in problem order; item
-// ranges per problem are in `start` (prefix sums, P+1 entries).  Each item
-// runs the body of the problem's list size K.  The counter is reset by a
-// kernel before every launch (no memset nodes in a captured graph).
-
-#define SNN_BATCH_SIZES(X) X(2) X(4) X(5) X(8) X(10) X(15) X(16) X(20) X(24)
-constexpr int kBatchMaxK = 24;
-constexpr int kBatchWaves = 8;  // dense batch block: 8 waves x 2 queries
+// Many independent problems per launch: a global counter hands out items
+// (problem, query block or query, candidate block) in problem order; the item
+// ranges of the problems are in `start` (prefix sums, P+1 entries).  The
+// problems of one launch share the list size K and the direction: registers
+// are allocated per kernel for its longest list, so one kernel for all list
+// sizes would run the short lists at the long list's occupancy.  The counter
+// is reset by a kernel before every launch (no memset nodes in a captured
+// graph).
 
 __device__ __forceinline__ int find_problem(const int64_t* start, int P, int64_t item) {
   int lo = 0, hi = P;  // largest p with start[p] <= item
@@ -435,14 +522,14 @@ __device__ __forceinline__ int64_t uniform64(int64_t v) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// Dense items of a problem: (candidate block, query block of 2*kBatchWaves
-// queries), candidate-block major.  All problems of one launch share MULT.
-template <int MULT, bool ZERO>
-__global__ __launch_bounds__(64 * kBatchWaves) void k_batch_dense(
+// Dense items of a problem: (candidate block, query block of QW*WAVES
+// queries), candidate-block major; the body of k_search_dense.
+template <int K, int QW, int WAVES, int MULT, bool ZERO>
+__global__ __launch_bounds__(64 * WAVES) void k_batch_dense(
     const SearchProblem* __restrict__ probs, const int64_t* __restrict__ start, int P,
     unsigned long long* __restrict__ counter) {
   constexpr int LDS_SCAN = 2 * 64 * 64;
-  constexpr int LDS_EPI = kBatchWaves * kBatchMaxK * 64;
+  constexpr int LDS_EPI = WAVES * K * 64;
   __shared__ float lds[LDS_SCAN > LDS_EPI ? LDS_SCAN : LDS_EPI];
   __shared__ long long s_item;
   const int64_t total = start[P];
@@ -455,29 +542,21 @@ __global__ __launch_bounds__(64 * kBatchWaves) void k_batch_dense(
     const int p = find_problem(start, P, item);
     const SearchProblem a = probs[p];
     const int64_t local = item - start[p];
-    const int64_t qblocks = (a.n + 2 * kBatchWaves - 1) / (2 * kBatchWaves);
-    const int cb = (int)(local / qblocks);
-    const int64_t qb = local % qblocks;
-    switch (search_list_size(a.kmax)) {
-#define SNN_CASE(KK) \
-  case KK: dense_block<KK, 2, kBatchWaves, MULT, ZERO>(a, lds, qb, cb); break;
-      SNN_BATCH_SIZES(SNN_CASE)
-#undef SNN_CASE
-      default: break;
-    }
+    const int64_t qblocks = (a.n + QW * WAVES - 1) / (QW * WAVES);
+    dense_block<K, QW, WAVES, MULT, ZERO>(a, lds, local % qblocks, (int)(local / qblocks));
   }
 }
 
 // Pruned items of a problem: (candidate block, query), candidate-block major;
 // one wave per item, no barriers.
-template <int MULT>
+template <int K, int MULT>
 __global__ __launch_bounds__(256) void k_batch_pruned(const SearchProblem* __restrict__ probs,
                                                       const int64_t* __restrict__ start, int P,
                                                       unsigned long long* __restrict__ counter) {
-  __shared__ float lds[4 * kBatchMaxK * 64];
+  __shared__ float lds[4 * K * 64];
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  float* lab = lds + wave * (kBatchMaxK * 64) + lane;
+  float* lab = lds + wave * (K * 64) + lane;
   const int64_t total = start[P];
   for (;;) {
     unsigned long long it = 0;
@@ -487,15 +566,7 @@ __global__ __launch_bounds__(256) void k_batch_pruned(const SearchProblem* __res
     const int p = find_problem(start, P, item);
     const SearchProblem a = probs[p];
     const int64_t local = item - start[p];
-    const int cb = (int)(local / a.n);
-    const int64_t q = local % a.n;
-    switch (search_list_size(a.kmax)) {
-#define SNN_CASE(KK) \
-  case KK: pruned_wave<KK, MULT>(a, lab, q, cb); break;
-      SNN_BATCH_SIZES(SNN_CASE)
-#undef SNN_CASE
-      default: break;
-    }
+    pruned_wave<K, MULT>(a, lab, local % a.n, (int)(local / a.n));
   }
 }
 
@@ -580,6 +651,23 @@ hipError_t launch_search_dense(const SearchProblem& a, hipStream_t s) {
   }
 }
 
+template <int K>
+static hipError_t launch_sorted_k(const SearchProblem& a, int max_cand, hipStream_t s) {
+  const dim3 g((unsigned)((a.n + 255) / 256), (unsigned)max_cand), b(256);
+  hipLaunchKernelGGL(k_search_sorted<K>, g, b, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_search_sorted(const SearchProblem& a, int max_cand, hipStream_t s) {
+  switch (search_list_size(a.kmax)) {
+#define SNN_CASE(K, QW, W) \
+  case K: return launch_sorted_k<K>(a, max_cand, s);
+    SNN_FOR_EACH_LIST_SIZE(SNN_CASE)
+#undef SNN_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_search_pruned(const SearchProblem& a, hipStream_t s) {
   switch (search_list_size(a.kmax)) {
 #define SNN_CASE(K, QW, W) \
@@ -590,48 +678,89 @@ hipError_t launch_search_pruned(const SearchProblem& a, hipStream_t s) {
   }
 }
 
-int batch_max_list_size() { return kBatchMaxK; }
+// Dense configuration (queries per wave, waves per block) of a list size.
+static void dense_config(int K, int* qw, int* waves) {
+  *qw = 1;
+  *waves = 4;
+  switch (K) {
+#define SNN_CASE(KK, QW, W) \
+  case KK: *qw = QW; *waves = W; break;
+    SNN_FOR_EACH_LIST_SIZE(SNN_CASE)
+#undef SNN_CASE
+    default: break;
+  }
+}
 
 int64_t batch_dense_items(const SearchProblem& a) {
-  return ((a.n + 2 * kBatchWaves - 1) / (2 * kBatchWaves)) * (a.cpad / 64);
+  int qw, w;
+  dense_config(search_list_size(a.kmax), &qw, &w);
+  return ((a.n + qw * w - 1) / (qw * w)) * (a.cpad / 64);
 }
 
 int64_t batch_pruned_items(const SearchProblem& a) { return a.n * (a.cpad / 64); }
 
-int batch_grid(int mode) {
-  // resident blocks per CU x CUs: the persistent grid fills the chip once
-  static int cached[2] = {0, 0};
-  int& g = cached[mode == 2 ? 1 : 0];
-  if (g) return g;
+// Resident blocks per CU x CUs: a persistent grid fills the chip once.
+template <class Kernel>
+static int resident_grid(Kernel kernel, int threads) {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     cus = 256;
-  hipError_t e;
-  if (mode == 2)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_pruned<1>, 256, 0);
-  else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_dense<1, false>,
-                                                     64 * kBatchWaves, 0);
-  if (e != hipSuccess || per < 1) per = 1;
-  g = cus * per;
-  return g;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess ||
+      per < 1)
+    per = 1;
+  return cus * per;
+}
+
+template <int K, int QW, int W>
+static int batch_grid_k(int mode, int mult) {
+  static const int g[4] = {
+      resident_grid(k_batch_dense<K, QW, W, 1, false>, 64 * W),
+      resident_grid(k_batch_dense<K, QW, W, 1, true>, 64 * W),
+      resident_grid(k_batch_pruned<K, 1>, 256), resident_grid(k_batch_pruned<K, -1>, 256)};
+  if (mode == 2) return g[mult > 0 ? 2 : 3];
+  return g[mode == 1 ? 1 : 0];
+}
+
+int batch_grid(int mode, int mult, int K) {
+  switch (K) {
+#define SNN_CASE(KK, QW, W) \
+  case KK: return batch_grid_k<KK, QW, W>(mode, mult);
+    SNN_FOR_EACH_LIST_SIZE(SNN_CASE)
+#undef SNN_CASE
+    default: return 0;
+  }
+}
+
+template <int K, int QW, int W>
+static void launch_batch_k(const SearchProblem* probs, const int64_t* start, int P,
+                           unsigned long long* counter, int mode, int mult, int grid,
+                           hipStream_t s) {
+  const dim3 g(grid), bp(256), bd(64 * W);
+  if (mode == 2) {
+    if (mult > 0) hipLaunchKernelGGL((k_batch_pruned<K, 1>), g, bp, 0, s, probs, start, P, counter);
+    else hipLaunchKernelGGL((k_batch_pruned<K, -1>), g, bp, 0, s, probs, start, P, counter);
+  } else if (mult < 0) {
+    hipLaunchKernelGGL((k_batch_dense<K, QW, W, -1, false>), g, bd, 0, s, probs, start, P,
+                       counter);
+  } else if (mode == 1) {
+    hipLaunchKernelGGL((k_batch_dense<K, QW, W, 1, true>), g, bd, 0, s, probs, start, P, counter);
+  } else {
+    hipLaunchKernelGGL((k_batch_dense<K, QW, W, 1, false>), g, bd, 0, s, probs, start, P,
+                       counter);
+  }
 }
 
 hipError_t launch_batch_search(const SearchProblem* probs, const int64_t* start, int P,
-                               unsigned long long* counter, int mode, int mult, int grid,
+                               unsigned long long* counter, int mode, int mult, int K, int grid,
                                hipStream_t s) {
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, counter);
-  const dim3 gp(grid), bp(256), bd(64 * kBatchWaves);
-  if (mode == 2) {
-    if (mult > 0) hipLaunchKernelGGL(k_batch_pruned<1>, gp, bp, 0, s, probs, start, P, counter);
-    else hipLaunchKernelGGL(k_batch_pruned<-1>, gp, bp, 0, s, probs, start, P, counter);
-  } else if (mult < 0) {
-    hipLaunchKernelGGL((k_batch_dense<-1, false>), gp, bd, 0, s, probs, start, P, counter);
-  } else if (mode == 1) {
-    hipLaunchKernelGGL((k_batch_dense<1, true>), gp, bd, 0, s, probs, start, P, counter);
-  } else {
-    hipLaunchKernelGGL((k_batch_dense<1, false>), gp, bd, 0, s, probs, start, P, counter);
+  switch (K) {
+#define SNN_CASE(KK, QW, W) \
+  case KK: launch_batch_k<KK, QW, W>(probs, start, P, counter, mode, mult, grid, s); break;
+    SNN_FOR_EACH_LIST_SIZE(SNN_CASE)
+#undef SNN_CASE
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

@@ -85,7 +85,7 @@ def test_batch_reports_invalid_problems_and_runs_the_rest(engine, mixed):
     bad = dict(probs[0])
     bad["k_fold"] = 0
     big_k = dict(probs[1])
-    big_k["k_vals"] = [32]
+    big_k["k_vals"] = [65]
     got, summary = engine.select_batch([bad, probs[0], big_k, probs[2]], form=1,
                                        raise_on_error=False)
     assert summary["rc"] == _lib.SNN_EINVAL
