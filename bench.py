@@ -1,19 +1,25 @@
-"""Benchmark: greedy kNN wrapper forward selection on one MI355X.
+"""Benchmark: greedy kNN wrapper feature selection on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2] [--form precision|plain]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c0|c2|c3|c4]
+                    [--form precision|plain] [--no-cpu-baseline]
+    python bench.py --copy-calibration BYTES   # a device copy, for counter scaling
 
-One bench step = one complete forward selection (every selection step:
-candidate gather, leave-one-out kNN search over all candidates, losses,
-arg-min, distance update) over the synthetic workload, with X and y already
-resident in HBM.  The default workload is n=20000 points, d=200 float32
-features, k=10, 3 classes, forward selection to 20 features (the benchmark's
-headline configuration).
+One bench step = one complete selection over the workload (every selection
+step: candidate gather, kNN search over all candidates with leave-one-out,
+losses, arg-min, distance update), with X and y already resident in HBM.  The
+default workload is forward selection on n=20000 points, d=200 float32
+features, k=10, 3 classes, to 20 features.  c2: n=65536, d=512, k=20, to 32
+features.  c3: 64 independent problems (n 1000..16000, d 32..256, k 5/10/20,
+to 8 features each) in one batched call per bench step, beside 64 sequential
+plain-form selections.  c4: backward elimination, n=32768, d=1024, k=15,
+regression, to 16 features.
 
 Printed: one JSON line with the whole-job throughput in (query x candidate)
-kNN evaluations per second, the selection wall time, the search kernel's
-fraction of the HBM roofline (algorithmic bytes 4*n per evaluation, timed
-with HIP events on the engine's stream) and the CPU oracle's rate on a
-bounded sample of the same workload.
+kNN evaluations per second, the selection wall time, the dominant search
+kernel's roofline (algorithmic bytes 4n per evaluation -- the query's D row --
+over the kernel's time from HIP events on the engine's stream; HBM traffic
+from the committed rocprofv3 counter summary when one exists) and the CPU
+oracle's rate on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -21,6 +27,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -30,6 +37,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "forward-selection wall s + (query x candidate) kNN evals/s; % HBM roofline"
+LIST_SIZES = [2, 4, 5, 8, 10, 15, 16, 20, 24, 32, 50, 64]
 
 
 class Hip:
@@ -42,13 +51,21 @@ class Hip:
         self.L.hipFree.argtypes = [C.c_void_p]
         self.L.hipDeviceSynchronize.argtypes = []
 
-    def upload(self, a: np.ndarray) -> int:
+    def alloc(self, nbytes: int) -> int:
         p = C.c_void_p()
-        if self.L.hipMalloc(C.byref(p), a.nbytes) != 0:
+        if self.L.hipMalloc(C.byref(p), nbytes) != 0:
             raise RuntimeError("hipMalloc failed")
-        if self.L.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) != 0:
-            raise RuntimeError("hipMemcpy failed")
         return p.value
+
+    def upload(self, a: np.ndarray) -> int:
+        p = self.alloc(a.nbytes)
+        if self.L.hipMemcpy(C.c_void_p(p), a.ctypes.data, a.nbytes, 1) != 0:
+            raise RuntimeError("hipMemcpy failed")
+        return p
+
+    def copy_d2d(self, dst: int, src: int, nbytes: int) -> None:
+        if self.L.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), nbytes, 3) != 0:
+            raise RuntimeError("hipMemcpy failed")
 
     def free(self, p: int) -> None:
         self.L.hipFree(C.c_void_p(p))
@@ -57,62 +74,77 @@ class Hip:
         self.L.hipDeviceSynchronize()
 
 
-def cpu_baseline(w: dict, target_s: float = 12.0) -> dict:
+def list_size(kmax: int) -> int:
+    return next(v for v in LIST_SIZES if kmax <= v)
+
+
+def traffic_per_launch(kernel_tag: str, workload: str):
+    """HBM bytes per launch from profiles/traffic_<kernel>_<workload>.md, or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel_tag}_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(X, y, kv, k_fold, model_type, method, target_s: float = 12.0) -> dict:
     """Oracle (oracle/oracle.c) rate on a bounded sample: the first selection
-    step (D = 0) for a prefix of the queries over every candidate."""
+    step for a prefix of the queries over every candidate."""
     from oracle import oracle as O
     O.build()
-    X, y = w["X"], w["y"]
     n, d = X.shape
     threads = max(1, min(16, os.cpu_count() or 1))
-    kv = w["k_vals"]
+    mult = 1 if method == 1 else -1
 
     def run(nq):
-        Drows = np.zeros((nq, n), np.float32)
         q = np.arange(nq, dtype=np.int64)
+        if mult > 0:
+            Drows = np.zeros((nq, n), np.float32)
+        else:  # backward start: full-d distance rows of the sampled queries
+            Drows = np.stack([O.distance_row(X, int(i), "backward", []) for i in q])
         t = time.perf_counter()
         for f in range(d):
-            O.candidate_preds(Drows, X, y, kv, w["k_fold"], f, 1, w["model_type"], queries=q,
+            O.candidate_preds(Drows, X, y, kv, k_fold, f, mult, model_type, queries=q,
                               threads=threads)
         return time.perf_counter() - t
 
-    nq = 64
+    nq = 16
     dt = run(nq)
-    nq = int(min(n, max(64, nq * target_s / max(dt, 1e-3))))
+    nq = int(min(n, max(16, nq * target_s / max(dt, 1e-3))))
     dt = run(nq)
+    step = "first forward step (D=0)" if mult > 0 else "first backward step (full D)"
     return {"value": nq * d / dt, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle, first forward step (D=0): {nq} queries x {d} candidates "
-                      f"x {n} points, {dt:.1f} s on {threads} threads"}
+            "sample": f"CPU oracle, {step}: {nq} queries x {d} candidates x {n} points, "
+                      f"{dt:.1f} s on {threads} threads"}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2"])
-    ap.add_argument("--form", default="precision", choices=["precision", "plain"])
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the selection "
-              "path is single-device here, see DESIGN.md for how it would shard.",
-              file=sys.stderr)
-        return 2
+def roofline(alg_bytes: float, ms: float, launches: int, kernel: str, traffic, unit_note: str):
+    achieved = alg_bytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
+           "launches": launches, "avg_launch_ms": ms / max(launches, 1),
+           "alg_bytes_per_launch": alg_bytes / max(launches, 1), "alg_bytes": unit_note}
+    if traffic is not None and ms > 0:
+        phys = traffic * launches / (ms / 1e3) / 1e9
+        out["physical_GBps"] = phys
+        out["physical_frac"] = phys / HBM_PEAK_GBS
+    return out
 
-    from speedynn_amd import _lib, datasets as ds
+
+def bench_single(args, hip, eng, form: int) -> dict:
+    from speedynn_amd import datasets as ds
     w = ds.config(args.workload)
     X, y = w["X"], w["y"]
     n, d = X.shape
-    hip = Hip()
-    eng = _lib.Engine(0)
+    method = 1 if w["method"] == "forward" else 0
+    mt = 1 if w["model_type"] == "classification" else 0
+    loss = 4 if mt == 1 else 0
     xp, yp = hip.upload(np.ascontiguousarray(X)), hip.upload(np.ascontiguousarray(y))
-    form = _lib.FORMS[args.form]
-    kw = dict(method=1, model_type=1, k_fold=w["k_fold"], num_features=w["num_features"],
+    kw = dict(method=method, model_type=mt, k_fold=w["k_fold"], num_features=w["num_features"],
               form=form, device_inputs=(xp, yp, n, d))
 
     def once():
-        return eng.select(None, None, w["k_vals"], [4], **kw)
+        return eng.select(None, None, w["k_vals"], [loss], **kw)
 
     for _ in range(args.warmup):
         once()
@@ -122,11 +154,26 @@ def main() -> int:
     hip.sync()
     wall = time.perf_counter() - t0
     r = results[-1]
-    evals = sum(x["evals"] for x in results)
-    search_ms = sum(x["ms_search"] for x in results)
     steps_sel = r["steps"]
-    alg_bytes = 4.0 * n * evals                       # one FP32 D row per evaluation
-    achieved = alg_bytes / (search_ms / 1e3) / 1e9    # GB/s over the search kernel's time
+    evals = sum(x["evals"] for x in results)
+
+    # dominant kernel: the pruned search (precision form; a forward selection's
+    # first step runs over sorted columns instead), else the dense search
+    K = list_size(max(w["k_vals"]))
+    if form == 1:
+        kernel = f"k_search_pruned<{K},{1 if method == 1 else -1}>"
+        first = 1 if method == 1 else 0
+        ms_dom = sum(x["ms_search"] - (x["ms_search_first"] if first else 0.0) for x in results)
+    else:
+        kernel = f"k_search_dense<{K}>"
+        first = 0
+        ms_dom = sum(x["ms_search"] for x in results)
+    dom_evals = args.steps * sum((d - t) * n for t in range(first, steps_sel))
+    launches = args.steps * (steps_sel - first)
+    tag = re.sub(r"[^A-Za-z0-9]+", "_", kernel).strip("_")
+    rf = roofline(4.0 * n * dom_evals, ms_dom, launches, kernel,
+                  traffic_per_launch(tag, args.workload),
+                  "4*n bytes (one FP32 D row) per (query x candidate) evaluation")
 
     parity = None
     gold = os.path.join(ROOT, "tests", "golden", f"{args.workload}_oracle.json")
@@ -138,40 +185,151 @@ def main() -> int:
         parity = bool(np.array_equal(r["selected_ordering"], exp["selected_ordering"]) and
                       np.array_equal(r["selected_errors"].view(np.uint32),
                                      np.array(exp["selected_errors_bits"], np.uint32)))
-
     line = {
-        "metric": "forward-selection wall s + (query x candidate) kNN evals/s; % HBM roofline",
-        "value": evals / wall,
-        "unit": "evals/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
+        "metric": METRIC, "value": evals / wall, "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (deterministic splitmix64 + Box-Muller generator)",
-        "config": {"workload": f"{args.workload}: forward selection n={n} d={d} "
-                               f"k={w['k_vals'][0]} LOO zero_one to {w['num_features']} features",
+        "config": {"workload": f"{args.workload}: {w['method']} selection n={n} d={d} "
+                               f"k={w['k_vals'][0]} LOO {'zero_one' if mt else 'rmse'} to "
+                               f"{w['num_features']} features",
                    "n": n, "d": d, "k": w["k_vals"][0], "features": w["num_features"],
                    "form": args.form, "parallelism": "single GPU"},
         "selection_wall_s": wall / args.steps,
         "selection_steps": steps_sel,
-        "search_ms_per_selection": search_ms / args.steps,
+        "search_ms_per_selection": sum(x["ms_search"] for x in results) / args.steps,
+        "first_step_search_ms": sum(x["ms_search_first"] for x in results) / args.steps,
+        "bound_ms_per_selection": sum(x["ms_bound"] for x in results) / args.steps,
         "update_ms_per_selection": sum(x["ms_update"] for x in results) / args.steps,
         "tally_ms_per_selection": sum(x["ms_tally"] for x in results) / args.steps,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_search (candidate search)",
-                     "alg_bytes_per_eval": 4 * n},
+        "distance_ms_per_selection": sum(x["ms_distance"] for x in results) / args.steps,
+        "pruned_point_fraction": (r["pruned_point_evals"] / (r["evals"] * (n - 1))
+                                  if r["evals"] else 0.0),
+        "roofline": rf,
         "parity_vs_golden": parity,
         "selected": r["selected_ordering"][:steps_sel].tolist(),
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(w)
+        line["cpu_baseline"] = cpu_baseline(X, y, w["k_vals"], w["k_fold"], mt, method)
     hip.free(xp)
     hip.free(yp)
+    return line
+
+
+def bench_batch(args, hip, eng, form: int) -> dict:
+    from speedynn_amd import datasets as ds
+    probs, ptrs = [], []
+    for p in range(64):
+        w = ds.batch_problem(p)
+        n, d = w["X"].shape
+        xp, yp = hip.upload(np.ascontiguousarray(w["X"])), hip.upload(np.ascontiguousarray(w["y"]))
+        ptrs += [xp, yp]
+        probs.append(dict(device_inputs=(xp, yp, n, d), X=w["X"], y=w["y"], k_vals=w["k_vals"],
+                          loss_codes=[4], method=1, model_type=1, k_fold=w["k_fold"],
+                          num_features=w["num_features"]))
+
+    def once(graph=True):
+        return eng.select_batch(probs, form=form, use_graph=graph)
+
+    for _ in range(args.warmup):
+        once()
+    hip.sync()
+    t0 = time.perf_counter()
+    runs = [once() for _ in range(args.steps)]
+    hip.sync()
+    wall = time.perf_counter() - t0
+    got, sm = runs[-1]
+    evals = sum(s["evals"] for _, s in runs)
+
+    # phase split from one more call without graphs (not part of the timed value)
+    _, split = once(graph=False)
+    first = 1 if form == 1 else 0
+    dom_bytes = 0.0  # 4*n_p per (query x candidate) evaluation of the dominant kernel
+    for p, g in zip(probs, got):
+        n, d = p["device_inputs"][2], p["device_inputs"][3]
+        dom_bytes += sum((d - t) * n * 4.0 * n for t in range(first, g["steps"]))
+    ms_dom = split["ms_search"] - (split["ms_search_first"] if first else 0.0)
+    sizes = sorted({list_size(max(p["k_vals"])) for p in probs})
+    launches = len(sizes) * (max(g["steps"] for g in got) - first)  # one per list size and step
+    family = "k_batch_pruned" if form == 1 else "k_batch_dense"
+    kernel = f"{family}<K> (K = {', '.join(map(str, sizes))})"
+    rf = roofline(dom_bytes, ms_dom, launches, kernel,
+                  traffic_per_launch(family, "c3"),
+                  "4*n_p bytes per (query x candidate) evaluation of problem p")
+
+    # the comparison: 64 sequential plain-form selections of the same problems
+    seq = None
+    if not args.no_sequential:
+        hip.sync()
+        t1 = time.perf_counter()
+        for p in probs:
+            xp, yp, n, d = p["device_inputs"]
+            eng.select(None, None, p["k_vals"], [4], method=1, model_type=1, k_fold=p["k_fold"],
+                       num_features=p["num_features"], form=0, device_inputs=(xp, yp, n, d))
+        hip.sync()
+        seq = time.perf_counter() - t1
+
+    line = {
+        "metric": METRIC, "value": evals / wall, "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (deterministic splitmix64 + Box-Muller generator)",
+        "config": {"workload": "c3: 64 forward selections in one batched call, n 1000..16000, "
+                               "d 32..256, k 5/10/20, LOO zero_one to 8 features",
+                   "problems": 64, "form": args.form, "graphs": True,
+                   "parallelism": "single GPU"},
+        "selection_wall_s": wall / args.steps,
+        "sequential_plain_s": seq,
+        "speedup_vs_sequential_plain": (seq / (wall / args.steps)) if seq else None,
+        "search_ms_per_batch": split["ms_search"],
+        "first_step_search_ms": split["ms_search_first"],
+        "bound_ms_per_batch": split["ms_bound"],
+        "roofline": rf,
+    }
+    if not args.no_cpu_baseline:
+        p0 = probs[0]
+        line["cpu_baseline"] = cpu_baseline(p0["X"], p0["y"], p0["k_vals"], p0["k_fold"], 1, 1)
+        line["cpu_baseline"]["sample"] += " (problem 0 of 64)"
+    for p in ptrs:
+        hip.free(p)
+    return line
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4"])
+    ap.add_argument("--form", default="precision", choices=["precision", "plain"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sequential", action="store_true", help="c3: skip the sequential runs")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="run device-to-device copies of BYTES (counter calibration) and exit")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the selection "
+              "engine runs on a single device.", file=sys.stderr)
+        return 2
+
+    hip = Hip()
+    if args.copy_calibration:
+        a, b = hip.alloc(args.copy_calibration), hip.alloc(args.copy_calibration)
+        for _ in range(3):
+            hip.copy_d2d(b, a, args.copy_calibration)
+        hip.sync()
+        hip.free(a)
+        hip.free(b)
+        print(json.dumps({"copy_bytes": args.copy_calibration, "copies": 3}), flush=True)
+        return 0
+
+    from speedynn_amd import _lib
+    eng = _lib.Engine(0)
+    form = _lib.FORMS[args.form]
+    if args.workload == "c3":
+        line = bench_batch(args, hip, eng, form)
+    else:
+        line = bench_single(args, hip, eng, form)
     eng.close()
     print(json.dumps(line), flush=True)
     return 0

@@ -108,6 +108,8 @@ typedef struct {
     double ms_tally;     /* candidate gather, losses and arg-min */
     int64_t evals;       /* (query x candidate) kNN evaluations performed */
     int64_t pruned_point_evals; /* point evaluations skipped by exact pruning */
+    double ms_bound;     /* pruning-bound seeds (part of the precision form's search) */
+    double ms_search_first; /* search kernels of the first step (included in ms_search) */
 } snn_summary;
 
 /* Create a context on HIP device `device`.  Returns SNN_OK or an error code. */
@@ -153,6 +155,7 @@ typedef struct {
     int num_final;               /* out */
     int optimal_k;               /* out */
     int steps;                   /* out */
+    int inputs_on_device;        /* X and y are device pointers of the context's device */
 } snn_problem;
 
 /* Many independent selections (uneven n, d and k) advanced together: every

@@ -37,7 +37,8 @@ class SnnSummary(C.Structure):
                 ("reserved", C.c_int), ("ms_total", C.c_double), ("ms_distance", C.c_double),
                 ("ms_search", C.c_double), ("ms_update", C.c_double), ("ms_tally", C.c_double),
                 ("evals", C.c_int64),
-                ("pruned_point_evals", C.c_int64)]
+                ("pruned_point_evals", C.c_int64), ("ms_bound", C.c_double),
+                ("ms_search_first", C.c_double)]
 
 
 class SnnProblem(C.Structure):
@@ -46,7 +47,7 @@ class SnnProblem(C.Structure):
                 ("num_loss", C.c_int), ("params", SnnParams),
                 ("selected_dimensions", C.c_void_p), ("selected_ordering", C.c_void_p),
                 ("selected_errors", C.c_void_p), ("status", C.c_int), ("num_final", C.c_int),
-                ("optimal_k", C.c_int), ("steps", C.c_int)]
+                ("optimal_k", C.c_int), ("steps", C.c_int), ("inputs_on_device", C.c_int)]
 
 
 class EngineError(RuntimeError):
@@ -171,7 +172,8 @@ class Engine:
                    selected_dimensions=sel, selected_ordering=order, selected_errors=errs,
                    ms_total=sm.ms_total, ms_distance=sm.ms_distance, ms_search=sm.ms_search,
                    ms_update=sm.ms_update, ms_tally=sm.ms_tally, evals=sm.evals,
-                   pruned_point_evals=sm.pruned_point_evals)
+                   pruned_point_evals=sm.pruned_point_evals, ms_bound=sm.ms_bound,
+                   ms_search_first=sm.ms_search_first)
         if cand is not None:
             out["candidate_errors"] = cand.reshape(d, d)
         return out
@@ -189,18 +191,24 @@ class Engine:
         arr = (SnnProblem * len(problems))()
         outs = []
         for i, pr in enumerate(problems):
-            X = np.ascontiguousarray(pr["X"], dtype=np.float32)
-            y = np.ascontiguousarray(pr["y"], dtype=np.float32).reshape(-1)
             kv = np.ascontiguousarray(pr["k_vals"], dtype=np.int32)
             lc = np.ascontiguousarray(pr["loss_codes"], dtype=np.int32)
-            n, d = X.shape
+            q = arr[i]
+            if pr.get("device_inputs") is not None:  # (X_ptr, y_ptr, n, d) in device memory
+                xp, yp, n, d = pr["device_inputs"]
+                q.X, q.y, q.inputs_on_device = xp, yp, 1
+            else:
+                X = np.ascontiguousarray(pr["X"], dtype=np.float32)
+                y = np.ascontiguousarray(pr["y"], dtype=np.float32).reshape(-1)
+                n, d = X.shape
+                keep += [X, y]
+                q.X, q.y = X.ctypes.data, y.ctypes.data
             sel = np.zeros(d, np.int32)
             order = np.zeros(d, np.int32)
             errs = np.zeros(len(lc) * d, np.float32)
-            keep += [X, y, kv, lc]
+            keep += [kv, lc]
             outs.append((sel, order, errs))
-            q = arr[i]
-            q.X, q.y, q.n, q.d = X.ctypes.data, y.ctypes.data, n, d
+            q.n, q.d = n, d
             q.k_vals, q.num_k = kv.ctypes.data, len(kv)
             q.loss_functions, q.num_loss = lc.ctypes.data, len(lc)
             q.params = SnnParams(method=int(pr["method"]), model_type=int(pr["model_type"]),
@@ -221,7 +229,8 @@ class Engine:
                                 steps=q.steps, selected_dimensions=sel,
                                 selected_ordering=order, selected_errors=errs))
         summary = dict(rc=rc, steps=sm.steps, ms_total=sm.ms_total, ms_distance=sm.ms_distance,
-                       ms_search=sm.ms_search, evals=sm.evals,
+                       ms_search=sm.ms_search, ms_search_first=sm.ms_search_first,
+                       ms_bound=sm.ms_bound, evals=sm.evals,
                        pruned_point_evals=sm.pruned_point_evals)
         return results, summary
 

@@ -36,6 +36,9 @@ struct Slot {
   bool has_med;      // median losses: sort space for NaN-holding error vectors
   bool sorted;       // forward, precision form: sorted columns for the first step
   size_t sort_bytes;
+  bool on_device;    // the caller's X and y are device pointers
+  const float* dX;   // X and y on the device (the caller's, or copies in the arena)
+  const float* dy;
 };
 
 size_t take(size_t& off, size_t bytes) {
@@ -95,26 +98,31 @@ struct Plan {
   // selection in the precision form: its first step, D = 0, runs over sorted
   // columns -- one launch per problem, a short step -- later ones run the
   // pruned search).
-  int issue_step(bool first_step, hipStream_t s, hipEvent_t ev_search0,
-                 hipEvent_t ev_search1) const {
+  // ev[0]: gather done, ev[1]: bound seeds done, ev[2]: search done (each may
+  // be null).
+  int issue_step(bool first_step, hipStream_t s, const hipEvent_t* ev) const {
     for (const Slot& sl : slots) {
       const Problem& pb = sl.pb;
-      HIPCHK(snn::launch_compact(at<float>(A(), sl.X), at<int>(A(), sl.cand),
+      HIPCHK(snn::launch_compact(sl.dX, at<int>(A(), sl.cand),
                                  at<SelState>(A(), sl.state), at<float>(A(), sl.Xc), pb.n, pb.d,
                                  pb.cpad, s));
     }
-    if (ev_search0) HIPCHK(hipEventRecord(ev_search0, s));
+    if (ev) HIPCHK(hipEventRecord(ev[0], s));
     auto* probs = at<snn::SearchProblem>(Mt(), probs_off);
     auto* counter = at<unsigned long long>(Mt(), counter_off);
+    auto pruned_group = [&](const Group& g) { return pruned && !(g.mult > 0 && first_step); };
+    for (const Group& g : groups) {
+      if (!pruned_group(g)) continue;
+      for (int i = g.first; i < g.first + g.count; ++i) {
+        const Slot& sl = slots[i];
+        HIPCHK(snn::launch_seed_scan(at<float>(A(), sl.D), at<SelState>(A(), sl.state),
+                                     at<int>(A(), sl.seeds), sl.pb.n, sl.pb.npf, s));
+      }
+    }
+    if (ev) HIPCHK(hipEventRecord(ev[1], s));
     for (const Group& g : groups) {
       if (!g.count) continue;
-      const bool use_pruned = pruned && !(g.mult > 0 && first_step);
-      if (use_pruned) {
-        for (int i = g.first; i < g.first + g.count; ++i) {
-          const Slot& sl = slots[i];
-          HIPCHK(snn::launch_seed_scan(at<float>(A(), sl.D), at<SelState>(A(), sl.state),
-                                       at<int>(A(), sl.seeds), sl.pb.n, sl.pb.npf, s));
-        }
+      if (pruned_group(g)) {
         HIPCHK(snn::launch_batch_search(probs + g.first, at<int64_t>(Mt(), g.pruned_start),
                                         g.count, counter, 2, g.mult, g.K, g.grid_pruned, s));
       } else if (pruned && g.mult > 0) {
@@ -128,11 +136,11 @@ struct Plan {
                                         g.count, counter, 0, g.mult, g.K, g.grid_dense, s));
       }
     }
-    if (ev_search1) HIPCHK(hipEventRecord(ev_search1, s));
+    if (ev) HIPCHK(hipEventRecord(ev[2], s));
     for (const Slot& sl : slots) {
       const Problem& pb = sl.pb;
       SelState* st = at<SelState>(A(), sl.state);
-      HIPCHK(snn::launch_loss(at<float>(A(), sl.P), at<float>(A(), sl.y), at<int>(A(), sl.losses),
+      HIPCHK(snn::launch_loss(at<float>(A(), sl.P), sl.dy, at<int>(A(), sl.losses),
                               st, at<float>(A(), sl.err), pb.n, pb.cpad, pb.nk, pb.nl, pb.d,
                               sl.has_med ? at<float>(A(), sl.med) : nullptr, s));
       HIPCHK(snn::launch_select(st, pb, at<int>(A(), sl.kvals), at<float>(A(), sl.err),
@@ -148,7 +156,7 @@ struct Plan {
 
   int capture(bool first_step, hipStream_t s, hipGraphExec_t* out) const {
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const int rc = issue_step(first_step, s, nullptr, nullptr);
+    const int rc = issue_step(first_step, s, nullptr);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) {
@@ -236,8 +244,9 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
   for (Slot& s : plan.slots) {
     const Problem& pb = s.pb;
     const size_t n = (size_t)pb.n, d = (size_t)pb.d, cpad = (size_t)pb.cpad;
-    s.X = take(off, sizeof(float) * n * d);
-    s.y = take(off, sizeof(float) * n);
+    s.on_device = probs[s.index].inputs_on_device != 0;
+    s.X = s.on_device ? 0 : take(off, sizeof(float) * n * d);
+    s.y = s.on_device ? 0 : take(off, sizeof(float) * n);
     s.XT = take(off, sizeof(float) * n * d);
     s.D = take(off, sizeof(float) * n * n);
     s.Dlo = pruned ? take(off, sizeof(float) * n * n) : 0;
@@ -285,6 +294,11 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
   HIPCHK(cache.arena.ensure(arena_bytes));
   HIPCHK(cache.meta.ensure(moff));
   hipStream_t s = ctx->stream;
+  for (Slot& sl : plan.slots) {
+    const snn_problem& q = probs[sl.index];
+    sl.dX = sl.on_device ? q.X : at<float>(cache.arena, sl.X);
+    sl.dy = sl.on_device ? q.y : at<float>(cache.arena, sl.y);
+  }
 
   // ---- host copies of the search descriptors and work queues
   std::vector<snn::SearchProblem>& hp = plan.hp;
@@ -301,7 +315,7 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     a.D = at<float>(cache.arena, sl.D);
     a.M = (pruned && pb.method == 0) ? at<float>(cache.arena, sl.M) : nullptr;
     a.Xc = at<float>(cache.arena, sl.Xc);
-    a.y = at<float>(cache.arena, sl.y);
+    a.y = sl.dy;
     a.kvals = at<int>(cache.arena, sl.kvals);
     a.st = at<SelState>(cache.arena, sl.state);
     a.seeds = pruned ? at<int>(cache.arena, sl.seeds) : nullptr;
@@ -334,10 +348,12 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
   for (const Slot& sl : plan.slots) {
     const snn_problem& q = probs[sl.index];
     const Problem& pb = sl.pb;
-    HIPCHK(hipMemcpyAsync(at<float>(cache.arena, sl.X), q.X, sizeof(float) * pb.n * pb.d,
-                          hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(at<float>(cache.arena, sl.y), q.y, sizeof(float) * pb.n,
-                          hipMemcpyHostToDevice, s));
+    if (!sl.on_device) {
+      HIPCHK(hipMemcpyAsync(at<float>(cache.arena, sl.X), q.X, sizeof(float) * pb.n * pb.d,
+                            hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(at<float>(cache.arena, sl.y), q.y, sizeof(float) * pb.n,
+                            hipMemcpyHostToDevice, s));
+    }
     HIPCHK(hipMemcpyAsync(at<int>(cache.arena, sl.kvals), q.k_vals, sizeof(int) * pb.nk,
                           hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(at<int>(cache.arena, sl.losses), q.loss_functions, sizeof(int) * pb.nl,
@@ -351,7 +367,7 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     float* XT = at<float>(cache.arena, sl.XT);
     float* D = at<float>(cache.arena, sl.D);
     float* Dlo = pruned ? at<float>(cache.arena, sl.Dlo) : nullptr;
-    HIPCHK(snn::launch_transpose(at<float>(cache.arena, sl.X), XT, pb.n, pb.d, s));
+    HIPCHK(snn::launch_transpose(sl.dX, XT, pb.n, pb.d, s));
     HIPCHK(snn::launch_feature_order(XT, pb.n, pb.d, at<double>(cache.arena, sl.key),
                                      at<int>(cache.arena, sl.perm), pruned, s));
     HIPCHK(snn::launch_init_state(at<SelState>(cache.arena, sl.state), pb,
@@ -388,7 +404,7 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
   // ---- steps
   int max_steps = 0;
   for (const Slot& sl : plan.slots) max_steps = std::max(max_steps, sl.pb.max_steps);
-  double ms_search = 0.0;
+  double ms_search = 0.0, ms_search_first = 0.0, ms_bound = 0.0;  // without graphs only
   if (use_graph) {
     std::vector<size_t> key{(size_t)cache.arena.p, (size_t)cache.meta.p, (size_t)form,
                             (size_t)B, (size_t)plan.dense_first};
@@ -412,16 +428,22 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     }
     for (int t = 0; t < max_steps; ++t) HIPCHK(hipGraphLaunch(t == 0 ? cache.first : cache.rest, s));
   } else {
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> marks;
+    // per step: gather done, seeds done, search done (phase split of the summary)
+    std::vector<hipEvent_t> marks;
     for (int t = 0; t < max_steps; ++t) {
-      hipEvent_t a = tm.make(), b = tm.make();
-      int rc = plan.issue_step(t == 0, s, a, b);
+      const hipEvent_t ev[3] = {tm.make(), tm.make(), tm.make()};
+      int rc = plan.issue_step(t == 0, s, ev);
       if (rc) return rc;
-      marks.emplace_back(a, b);
+      marks.insert(marks.end(), ev, ev + 3);
     }
     HIPCHK(hipEventRecord(e2, s));
     HIPCHK(hipStreamSynchronize(s));
-    for (auto& m : marks) ms_search += elapsed(m.first, m.second);
+    for (size_t i = 0; i + 2 < marks.size(); i += 3) {
+      ms_bound += elapsed(marks[i], marks[i + 1]);
+      const double sm = elapsed(marks[i + 1], marks[i + 2]);
+      ms_search += sm;
+      if (i == 0) ms_search_first = sm;
+    }
   }
   if (use_graph) {
     HIPCHK(hipEventRecord(e2, s));
@@ -464,6 +486,8 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     summary->ms_total = elapsed(e0, e2);
     summary->ms_distance = elapsed(e0, e1);
     summary->ms_search = ms_search;
+    summary->ms_search_first = ms_search_first;
+    summary->ms_bound = ms_bound;
     summary->evals = evals;
     if (pruned) {
       unsigned long long done = 0;

@@ -236,15 +236,19 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     sl.visited = ctx->visited.as<unsigned long long>();
   }
 
-  // per step: [a] gather [b] search [c] losses + arg-min [u] update [next a]
+  // per step: [a] gather [b] bound seeds [b2] search [c] losses + arg-min
+  // [u] update [next a]
+  constexpr int kMarks = 5;
   std::vector<hipEvent_t> marks;
   for (int step = 0; step < pb.max_steps; ++step) {
-    hipEvent_t a = tm.make(), b = tm.make(), c = tm.make(), u = tm.make();
+    hipEvent_t a = tm.make(), b = tm.make(), b2 = tm.make(), c = tm.make(), u = tm.make();
     HIPCHK(hipEventRecord(a, s));
     HIPCHK(snn::launch_compact(dX, ctx->cand.as<int>(), st, ctx->Xc.as<float>(), n, d, pb.cpad, s));
     HIPCHK(hipEventRecord(b, s));
-    if (pruned && (step > 0 || !fwd)) {
-      HIPCHK(snn::launch_seed_scan(D, st, ctx->seeds.as<int>(), n, pb.npf, s));
+    const bool pruned_step = pruned && (step > 0 || !fwd);
+    if (pruned_step) HIPCHK(snn::launch_seed_scan(D, st, ctx->seeds.as<int>(), n, pb.npf, s));
+    HIPCHK(hipEventRecord(b2, s));
+    if (pruned_step) {
       HIPCHK(snn::launch_search_pruned(sl, s));
     } else {
       // first forward step (D = 0): sorted columns; the dense search runs
@@ -262,10 +266,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
                               cand_err, ctx->perm.as<int>(), s));
     HIPCHK(hipEventRecord(u, s));
     HIPCHK(snn::launch_update(D, Dlo, ctx->XT.as<float>(), n, st, -1, fwd ? 1 : -1, s));
-    marks.push_back(a);
-    marks.push_back(b);
-    marks.push_back(c);
-    marks.push_back(u);
+    for (hipEvent_t e : {a, b, b2, c, u}) marks.push_back(e);
   }
   hipEvent_t e2 = tm.make();
   HIPCHK(hipEventRecord(e2, s));
@@ -291,16 +292,20 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     summary->steps = hs.step;
     summary->ms_total = elapsed(e0, e2);
     summary->ms_distance = elapsed(e0, e1);
-    double ms_search = 0, ms_update = 0, ms_tally = 0;
-    for (size_t i = 0; i + 3 < marks.size(); i += 4) {
-      ms_tally += elapsed(marks[i], marks[i + 1]) + elapsed(marks[i + 2], marks[i + 3]);
-      ms_search += elapsed(marks[i + 1], marks[i + 2]);
-      const hipEvent_t next = (i + 4 < marks.size()) ? marks[i + 4] : e2;
-      ms_update += elapsed(marks[i + 3], next);
+    double ms_search = 0, ms_update = 0, ms_tally = 0, ms_bound = 0;
+    for (size_t i = 0; i + kMarks - 1 < marks.size(); i += kMarks) {
+      ms_tally += elapsed(marks[i], marks[i + 1]) + elapsed(marks[i + 3], marks[i + 4]);
+      ms_bound += elapsed(marks[i + 1], marks[i + 2]);
+      const double sm = elapsed(marks[i + 2], marks[i + 3]);
+      ms_search += sm;
+      if (i == 0) summary->ms_search_first = sm;
+      const hipEvent_t next = (i + kMarks < marks.size()) ? marks[i + kMarks] : e2;
+      ms_update += elapsed(marks[i + 4], next);
     }
     summary->ms_search = ms_search;
     summary->ms_update = ms_update;
     summary->ms_tally = ms_tally;
+    summary->ms_bound = ms_bound;
     // candidates evaluated per step: fwd d, d-1, ...; bwd d, d-1, ...
     int64_t ev = 0, pruned_full = 0;
     for (int t = 0; t < hs.step; ++t) {
