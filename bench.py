@@ -161,7 +161,15 @@ def bench_single(args, hip, eng, form: int) -> dict:
     # first step runs over sorted columns instead), else the dense search
     K = list_size(max(w["k_vals"]))
     if form == 1:
-        kernel = f"k_search_pruned<{K},{1 if method == 1 else -1}>"
+        mult = 1 if method == 1 else -1
+        nblk = (d + 63) // 64
+        W = 4 if nblk <= 4 else 8 if nblk <= 8 else 16
+        names = []  # the kernels the autotuner kept for this workload's steps
+        if r["pruned_kernel"] & 1:
+            names.append(f"k_search_pruned<{K},{mult}>")
+        if r["pruned_kernel"] & 2:
+            names.append(f"k_search_pruned_shared<{K},{mult},{W}>")
+        kernel = " + ".join(names) or f"k_search_pruned<{K},{mult}>"
         first = 1 if method == 1 else 0
         ms_dom = sum(x["ms_search"] - (x["ms_search_first"] if first else 0.0) for x in results)
     else:
@@ -295,6 +303,33 @@ def bench_batch(args, hip, eng, form: int) -> dict:
     return line
 
 
+def bench_gram(args, hip, eng) -> dict:
+    """(a) alone: the full-d distance matrix of a workload's X, exact FP32 in
+    the reference's order (VALU) and the BF16x3 MFMA Gram; best of 3 after a
+    warm-up, HIP events around the matrix kernels."""
+    from speedynn_amd import datasets as ds
+    w = ds.config(args.gram)
+    X = np.ascontiguousarray(w["X"])
+    n, d = X.shape
+    xp, dp = hip.upload(X), hip.alloc(4 * n * n)
+    times = {}
+    for form, name in ((2, "mfma"), (0, "exact")):
+        eng.distance_matrix(xp, n, d, form, dp)
+        times[name] = min(eng.distance_matrix(xp, n, d, form, dp) for _ in range(3))
+    hip.free(xp)
+    hip.free(dp)
+    flops = 2.0 * n * n * d  # algorithmic: one multiply-add per (pair, feature)
+    t_m, t_e = times["mfma"] / 1e3, times["exact"] / 1e3
+    return {"gram": args.gram, "n": n, "d": d,
+            "mfma_ms": times["mfma"], "exact_ms": times["exact"],
+            "mfma_alg_TFLOPs": flops / t_m / 1e12,
+            "mfma_util": 6 * flops / t_m / 2.5e15,  # six bf16 products per term
+            "mfma_write_GBps": 4.0 * n * n / t_m / 1e9,
+            "exact_alg_TFLOPs": flops / t_e / 1e12,
+            "note": "mfma_util counts the six split products against the 2.5 PFLOP/s dense "
+                    "BF16 peak; D write is 4n^2 bytes"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -306,6 +341,8 @@ def main() -> int:
     ap.add_argument("--no-sequential", action="store_true", help="c3: skip the sequential runs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="run device-to-device copies of BYTES (counter calibration) and exit")
+    ap.add_argument("--gram", choices=["c1", "c2", "c4"], default=None,
+                    help="time only the full-d distance matrix of a workload and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the selection "
@@ -326,6 +363,10 @@ def main() -> int:
     from speedynn_amd import _lib
     eng = _lib.Engine(0)
     form = _lib.FORMS[args.form]
+    if args.gram:
+        print(json.dumps(bench_gram(args, hip, eng)), flush=True)
+        eng.close()
+        return 0
     if args.workload == "c3":
         line = bench_batch(args, hip, eng, form)
     else:

@@ -81,7 +81,8 @@ float get_loss_all(float *labels, int num_labels, float *predictions, int num_pr
 
 /* Precision forms. */
 #define SNN_FORM_PLAIN 0     /* FP32 throughout, brute-force search */
-#define SNN_FORM_PRECISION 1 /* compensated D, screened + pruned search, exact recheck */
+#define SNN_FORM_PRECISION 1 /* sorted first step, exact pruned + screened search */
+#define SNN_FORM_MFMA 2      /* distance matrices only: BF16x3 MFMA Gram (approximate) */
 
 typedef struct snn_ctx snn_ctx;
 
@@ -92,15 +93,18 @@ typedef struct {
     int num_features;    /* desired features (ignored when auto_feature) */
     int auto_feature;    /* stop at the first error increase */
     int form;            /* SNN_FORM_* */
-    int use_graph;       /* capture one selection step in a hipGraph and replay it */
-    int reserved;
+    int use_graph;       /* snn_select_batch: capture one selection step in a hipGraph */
+    int compensated;     /* precision form: also keep D's compensation term (TwoSum /
+                            TwoProduct error, one more n*n float array); the selection
+                            does not depend on it */
 } snn_params;
 
 typedef struct {
     int num_final;       /* return value of train_*_selection */
     int optimal_k;       /* get_final_optimal_model_k */
     int steps;           /* selection steps executed */
-    int reserved;
+    int pruned_kernel;   /* pruned searches used, bit mask: 1 a wave per (query,
+                            candidate block), 2 a workgroup per query sharing the row scan */
     double ms_total;     /* device time of the selection (HIP events) */
     double ms_distance;  /* (a) initial matrix */
     double ms_search;    /* (c) candidate search kernels only */
@@ -179,6 +183,16 @@ int snn_select_batch(snn_ctx *ctx, int num_problems, snn_problem *problems,
 int snn_distance_rows(snn_ctx *ctx, const float *X, int64_t n, int d, int method,
                       const int *order, int s, int form, const int64_t *rows, int nrows,
                       float *rows_out, float *lo_out);
+
+/* The full-d distance matrix of X (device, n x d row-major) into D_device
+ * (n x n floats): form SNN_FORM_PLAIN = exact FP32 in the reference's
+ * descending-feature order (the selection's backward start), SNN_FORM_MFMA =
+ * the BF16x3 matrix-core Gram over centred features (within 1e-5 relative of
+ * FP64, not the reference's bits).  ms (optional) receives the device time of
+ * the matrix kernels.  Replaces compute_distance_matrix_gpu,
+ * speedynn/lib/gpu.c:354-379 (and compute_distance_matrix_cpu, cpu.c:185-215). */
+int snn_distance_matrix(snn_ctx *ctx, const float *X_device, int64_t n, int d, int form,
+                        float *D_device, double *ms);
 
 /* Loss of predictions against labels on the GPU (any SNN_LOSS_*). */
 int snn_loss(snn_ctx *ctx, const float *labels, const float *predictions, int64_t n,

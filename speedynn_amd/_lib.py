@@ -23,18 +23,20 @@ EXPORTS = [
     "get_selected_ordering", "get_selected_errors", "get_final_optimal_model_k", "get_loss_all",
     "snn_create", "snn_destroy", "snn_last_error", "snn_select", "snn_select_device",
     "snn_distance_rows", "snn_loss", "snn_set_default_form", "snn_select_batch",
+    "snn_distance_matrix",
 ]
 
 
 class SnnParams(C.Structure):
     _fields_ = [("method", C.c_int), ("model_type", C.c_int), ("k_fold", C.c_int64),
                 ("num_features", C.c_int), ("auto_feature", C.c_int), ("form", C.c_int),
-                ("use_graph", C.c_int), ("reserved", C.c_int)]
+                ("use_graph", C.c_int), ("compensated", C.c_int)]
 
 
 class SnnSummary(C.Structure):
     _fields_ = [("num_final", C.c_int), ("optimal_k", C.c_int), ("steps", C.c_int),
-                ("reserved", C.c_int), ("ms_total", C.c_double), ("ms_distance", C.c_double),
+                ("pruned_kernel", C.c_int), ("ms_total", C.c_double),
+                ("ms_distance", C.c_double),
                 ("ms_search", C.c_double), ("ms_update", C.c_double), ("ms_tally", C.c_double),
                 ("evals", C.c_int64),
                 ("pruned_point_evals", C.c_int64), ("ms_bound", C.c_double),
@@ -90,6 +92,8 @@ def lib() -> C.CDLL:
                                C.POINTER(C.c_float)]
         L.snn_set_default_form.argtypes = [C.c_int]
         L.snn_set_default_form.restype = None
+        L.snn_distance_matrix.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int,
+                                          C.c_void_p, C.POINTER(C.c_double)]
         L.snn_select_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(SnnProblem),
                                        C.POINTER(SnnSummary)]
         i = C.c_int
@@ -141,7 +145,8 @@ class Engine:
 
     def select(self, X, y, k_vals, loss_codes, *, method: int, model_type: int, k_fold: int,
                num_features: int, auto_feature: bool = False, form: int = 1,
-               candidate_errors: bool = False, device_inputs=None) -> dict:
+               candidate_errors: bool = False, device_inputs=None,
+               compensated: bool = False) -> dict:
         """Run one selection; device_inputs=(X_ptr, y_ptr, n, d) skips the upload."""
         kv = np.ascontiguousarray(k_vals, dtype=np.int32)
         lc = np.ascontiguousarray(loss_codes, dtype=np.int32)
@@ -153,7 +158,7 @@ class Engine:
             xp, yp, n, d = device_inputs
         p = SnnParams(method=int(method), model_type=int(model_type), k_fold=int(k_fold),
                       num_features=int(num_features), auto_feature=int(bool(auto_feature)),
-                      form=int(form), use_graph=0, reserved=0)
+                      form=int(form), use_graph=0, compensated=int(bool(compensated)))
         sel = np.zeros(d, np.int32)
         order = np.zeros(d, np.int32)
         errs = np.zeros(len(lc) * d, np.float32)
@@ -173,6 +178,7 @@ class Engine:
                    ms_total=sm.ms_total, ms_distance=sm.ms_distance, ms_search=sm.ms_search,
                    ms_update=sm.ms_update, ms_tally=sm.ms_tally, evals=sm.evals,
                    pruned_point_evals=sm.pruned_point_evals, ms_bound=sm.ms_bound,
+                   pruned_kernel=sm.pruned_kernel,
                    ms_search_first=sm.ms_search_first)
         if cand is not None:
             out["candidate_errors"] = cand.reshape(d, d)
@@ -214,7 +220,8 @@ class Engine:
             q.params = SnnParams(method=int(pr["method"]), model_type=int(pr["model_type"]),
                                  k_fold=int(pr["k_fold"]), num_features=int(pr["num_features"]),
                                  auto_feature=int(bool(pr.get("auto_feature", False))),
-                                 form=int(form), use_graph=int(bool(use_graph)), reserved=0)
+                                 form=int(form), use_graph=int(bool(use_graph)),
+                                 compensated=int(bool(pr.get("compensated", False))))
             q.selected_dimensions = sel.ctypes.data
             q.selected_ordering = order.ctypes.data
             q.selected_errors = errs.ctypes.data
@@ -245,6 +252,15 @@ class Engine:
                                       len(r), out.reshape(-1),
                                       lo.ctypes.data if lo is not None else None))
         return out, lo
+
+    def distance_matrix(self, X_ptr: int, n: int, d: int, form: int, D_ptr: int) -> float:
+        """Full-d distance matrix of device-resident X into device D (n*n
+        floats); form 0 = exact FP32 in the reference's order, 2 = BF16x3
+        MFMA Gram.  Returns the kernels' device time in ms."""
+        ms = C.c_double()
+        check(lib().snn_distance_matrix(self._ctx, C.c_void_p(X_ptr), int(n), int(d), int(form),
+                                        C.c_void_p(D_ptr), C.byref(ms)))
+        return ms.value
 
     def loss(self, labels, preds, code: int) -> float:
         y = np.ascontiguousarray(labels, dtype=np.float32)

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libspeedynn_amd.so")
-SOURCES = ["kernels.hip", "search.hip", "sort.hip", "engine.hip", "batch.hip"]
+SOURCES = ["kernels.hip", "search.hip", "sort.hip", "gram.hip", "engine.hip", "batch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPEEDYNN_ARCH", "gfx950")
 

@@ -36,6 +36,7 @@ struct Slot {
   bool has_med;      // median losses: sort space for NaN-holding error vectors
   bool sorted;       // forward, precision form: sorted columns for the first step
   size_t sort_bytes;
+  bool comp;         // keep D's compensation term
   bool on_device;    // the caller's X and y are device pointers
   const float* dX;   // X and y on the device (the caller's, or copies in the arena)
   const float* dy;
@@ -148,7 +149,7 @@ struct Plan {
                                 at<int>(A(), sl.ordering), at<float>(A(), sl.errors),
                                 at<float>(A(), sl.val), at<int>(A(), sl.bestk), nullptr,
                                 at<int>(A(), sl.perm), s));
-      HIPCHK(snn::launch_update(at<float>(A(), sl.D), pruned ? at<float>(A(), sl.Dlo) : nullptr,
+      HIPCHK(snn::launch_update(at<float>(A(), sl.D), sl.comp ? at<float>(A(), sl.Dlo) : nullptr,
                                 at<float>(A(), sl.XT), pb.n, st, -1, pb.method == 1 ? 1 : -1, s));
     }
     return SNN_OK;
@@ -249,7 +250,8 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     s.y = s.on_device ? 0 : take(off, sizeof(float) * n);
     s.XT = take(off, sizeof(float) * n * d);
     s.D = take(off, sizeof(float) * n * n);
-    s.Dlo = pruned ? take(off, sizeof(float) * n * n) : 0;
+    s.comp = pruned && probs[s.index].params.compensated;
+    s.Dlo = s.comp ? take(off, sizeof(float) * n * n) : 0;
     s.M = (pruned && pb.method == 0) ? take(off, sizeof(float) * n * n) : 0;
     s.Xc = take(off, sizeof(float) * n * cpad);
     s.P = take(off, sizeof(float) * pb.nk * cpad * n);
@@ -329,6 +331,7 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     a.mult = pb.method == 1 ? 1 : -1;
     a.model_type = pb.model_type;
     a.zero = 0;
+    a.nseed = snn::backward_seed_count(pb.kmax);
   }
   std::memcpy(hmeta.data() + plan.probs_off, hp.data(), sizeof(snn::SearchProblem) * B);
   for (const Group& g : plan.groups) {
@@ -366,7 +369,7 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     const int64_t nn = pb.n * pb.n;
     float* XT = at<float>(cache.arena, sl.XT);
     float* D = at<float>(cache.arena, sl.D);
-    float* Dlo = pruned ? at<float>(cache.arena, sl.Dlo) : nullptr;
+    float* Dlo = sl.comp ? at<float>(cache.arena, sl.Dlo) : nullptr;
     HIPCHK(snn::launch_transpose(sl.dX, XT, pb.n, pb.d, s));
     HIPCHK(snn::launch_feature_order(XT, pb.n, pb.d, at<double>(cache.arena, sl.key),
                                      at<int>(cache.arena, sl.perm), pruned, s));
@@ -408,10 +411,12 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
   if (use_graph) {
     std::vector<size_t> key{(size_t)cache.arena.p, (size_t)cache.meta.p, (size_t)form,
                             (size_t)B, (size_t)plan.dense_first};
+    // everything a captured launch embeds: shapes, problem order, device pointers
     for (const Slot& sl : plan.slots) {
       const Problem& pb = sl.pb;
-      const int64_t f[] = {pb.n, pb.npf, pb.d, pb.nk, pb.nl, pb.kmax, pb.method, pb.model_type,
-                           pb.auto_feature, pb.max_steps, pb.num_desired};
+      const int64_t f[] = {sl.index, pb.n, pb.npf, pb.d, pb.nk, pb.nl, pb.kmax, pb.method,
+                           pb.model_type, pb.auto_feature, pb.max_steps, pb.num_desired,
+                           sl.comp, sl.has_med, (int64_t)(size_t)sl.dX, (int64_t)(size_t)sl.dy};
       for (int64_t v : f) key.push_back((size_t)v);
     }
     if (key != cache.key || old_arena != cache.arena.p || old_meta != cache.meta.p ||

@@ -69,6 +69,7 @@ struct SearchProblem {
   int model_type;
   int zero;              // D is known to be 0 (first forward step)
   int sorted;            // first forward step: sorted columns serve finite X
+  int nseed;             // backward pruned search: seeds read for the bound (<= 128)
   const float* skey;     // [d][n] each feature's values, ascending
   const int* sidx;       // [d][n] their point indices
   const int* cand;       // [d] feature of each candidate slot

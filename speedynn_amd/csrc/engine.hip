@@ -9,8 +9,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/speedynn.h"
@@ -115,6 +117,60 @@ double elapsed(hipEvent_t a, hipEvent_t b) {
   return 0.0;
 }
 
+// Autotuner of the pruned search.  Which kernel is faster -- a wave per
+// (query, candidate block), or a workgroup per query sharing the row scan --
+// depends on the data (how much the candidate blocks' bounds differ) and on
+// the step (early steps keep many points, late ones few), not only on the
+// shape.  The first step of a shape (n, d, list size, direction) and step
+// range [2^b, 2^(b+1)) in this process runs both on the same step, timed
+// with HIP events (their results are bit-identical; the first run does not
+// count visits), and the faster serves the rest of that range.
+// SPEEDYNN_TUNING=off keeps the rule by direction.
+std::mutex g_tune_mu;
+std::map<std::tuple<int64_t, int, int, int, int>, int> g_tuned;
+
+int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, hipStream_t s,
+                        int* used) {
+  static const bool off = [] {
+    const char* e = std::getenv("SPEEDYNN_TUNING");
+    return e && !std::strcmp(e, "off");
+  }();
+  int bucket = 0;
+  while ((2 << bucket) <= step + 1) ++bucket;
+  const auto key = std::make_tuple(sl.n, d, snn::search_list_size(sl.kmax), sl.mult, bucket);
+  int variant = 0;
+  bool tune = false;
+  if (!off && snn::pruned_shared_applies(sl)) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) variant = it->second;
+    else tune = true;
+  }
+  if (!tune) {
+    HIPCHK(snn::launch_search_pruned(sl, variant, s));
+    const int v = variant ? variant : (sl.mult > 0 || !snn::pruned_shared_applies(sl) ? 1 : 2);
+    *used |= 1 << (v - 1);
+    return SNN_OK;
+  }
+  Timing tm;
+  hipEvent_t t0 = tm.make(), t1 = tm.make(), t2 = tm.make();
+  snn::SearchProblem quiet = sl;
+  quiet.visited = nullptr;
+  HIPCHK(hipEventRecord(t0, s));
+  HIPCHK(snn::launch_search_pruned(quiet, 1, s));
+  HIPCHK(hipEventRecord(t1, s));
+  HIPCHK(snn::launch_search_pruned(sl, 2, s));
+  HIPCHK(hipEventRecord(t2, s));
+  HIPCHK(hipEventSynchronize(t2));
+  variant = elapsed(t1, t2) < elapsed(t0, t1) ? 2 : 1;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tuned[key] = variant;
+  }
+  *used |= 3;  // this step ran both
+  return SNN_OK;
+}
+
 int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy,
                 const int* k_vals, int num_k, const int* loss_functions, int num_loss,
                 const snn_params* prm, int* selected_dimensions, int* selected_ordering,
@@ -125,7 +181,11 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   if (rc) return rc;
   const int kmax = pb.kmax;
   const bool fwd = prm->method == 1;
-  const bool comp = prm->form == SNN_FORM_PRECISION;
+  // precision form: exact pruned search over D in the reference's bits; the
+  // compensation term Dlo of D is kept only on request (nothing in the
+  // selection reads it)
+  const bool pruned = prm->form == SNN_FORM_PRECISION;
+  const bool comp = pruned && prm->compensated;
 
   HIPCHK(hipSetDevice(ctx->device));
   const size_t nn = (size_t)n * (size_t)n;
@@ -146,9 +206,8 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   HIPCHK(ctx->state.ensure(sizeof(SelState)));
   HIPCHK(ctx->perm.ensure(sizeof(int) * d));
   HIPCHK(ctx->key.ensure(sizeof(double) * d));
-  // precision form, forward: exact pruning with bounds from 128 seeds per query
-  // (backward also keeps M, the max single-feature term, for its lower bound)
-  const bool pruned = comp;
+  // exact pruning with bounds from 128 seeds per query (backward also keeps M,
+  // the max single-feature term, for its lower bound)
   if (pruned && !fwd) HIPCHK(ctx->M.ensure(sizeof(float) * nn));
   if (pruned) {
     HIPCHK(ctx->seeds.ensure(sizeof(int) * (size_t)n * 128));
@@ -232,6 +291,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   }
   if (pruned) {
     sl.M = fwd ? nullptr : ctx->M.as<float>();
+    sl.nseed = snn::backward_seed_count(kmax);
     sl.seeds = ctx->seeds.as<int>();
     sl.visited = ctx->visited.as<unsigned long long>();
   }
@@ -239,6 +299,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   // per step: [a] gather [b] bound seeds [b2] search [c] losses + arg-min
   // [u] update [next a]
   constexpr int kMarks = 5;
+  int used_variant = 0;
   std::vector<hipEvent_t> marks;
   for (int step = 0; step < pb.max_steps; ++step) {
     hipEvent_t a = tm.make(), b = tm.make(), b2 = tm.make(), c = tm.make(), u = tm.make();
@@ -249,7 +310,8 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     if (pruned_step) HIPCHK(snn::launch_seed_scan(D, st, ctx->seeds.as<int>(), n, pb.npf, s));
     HIPCHK(hipEventRecord(b2, s));
     if (pruned_step) {
-      HIPCHK(snn::launch_search_pruned(sl, s));
+      int rc2 = search_pruned_tuned(sl, d, step, s, &used_variant);
+      if (rc2) return rc2;
     } else {
       // first forward step (D = 0): sorted columns; the dense search runs
       // instead where X is not finite (each kernel checks the state's flag)
@@ -290,6 +352,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     summary->num_final = hs.num_final;
     summary->optimal_k = hs.best_k;
     summary->steps = hs.step;
+    summary->pruned_kernel = used_variant;
     summary->ms_total = elapsed(e0, e2);
     summary->ms_distance = elapsed(e0, e1);
     double ms_search = 0, ms_update = 0, ms_tally = 0, ms_bound = 0;
@@ -450,6 +513,8 @@ int snn_distance_rows(snn_ctx* ctx, const float* X, int64_t n, int d, int method
     if (order[t] < 0 || order[t] >= d) return fail(SNN_EINVAL, "feature index out of range");
   for (int r = 0; r < nrows; ++r)
     if (rows[r] < 0 || rows[r] >= n) return fail(SNN_EINVAL, "row index out of range");
+  if (form != SNN_FORM_PLAIN && form != SNN_FORM_PRECISION && form != SNN_FORM_MFMA)
+    return fail(SNN_EINVAL, "unknown form");
   const bool comp = form == SNN_FORM_PRECISION;
   HIPCHK(hipSetDevice(ctx->device));
   const size_t nn = (size_t)n * (size_t)n;
@@ -457,6 +522,7 @@ int snn_distance_rows(snn_ctx* ctx, const float* X, int64_t n, int d, int method
   HIPCHK(ctx->XT.ensure(sizeof(float) * (size_t)n * d));
   HIPCHK(ctx->D.ensure(sizeof(float) * nn));
   if (comp) HIPCHK(ctx->Dlo.ensure(sizeof(float) * nn));
+  if (form == SNN_FORM_MFMA) HIPCHK(ctx->scratch.ensure(snn::gram_mfma_scratch_bytes(n, d)));
   hipStream_t st = ctx->stream;
   float* D = ctx->D.as<float>();
   float* Dlo = comp ? ctx->Dlo.as<float>() : nullptr;
@@ -465,6 +531,9 @@ int snn_distance_rows(snn_ctx* ctx, const float* X, int64_t n, int d, int method
   if (method == 1) {
     HIPCHK(snn::launch_fill(D, (int64_t)nn, 0.0f, st));
     if (Dlo) HIPCHK(snn::launch_fill(Dlo, (int64_t)nn, 0.0f, st));
+  } else if (form == SNN_FORM_MFMA) {
+    HIPCHK(snn::launch_gram_mfma(ctx->X.as<float>(), ctx->XT.as<float>(), n, d, D,
+                                 ctx->scratch.p, st));
   } else {
     HIPCHK(snn::launch_gram_exact(ctx->XT.as<float>(), D, Dlo, nullptr, n, d, st));
   }
@@ -479,6 +548,31 @@ int snn_distance_rows(snn_ctx* ctx, const float* X, int64_t n, int d, int method
                             hipMemcpyDeviceToHost, st));
   }
   HIPCHK(hipStreamSynchronize(st));
+  return SNN_OK;
+}
+
+int snn_distance_matrix(snn_ctx* ctx, const float* X_device, int64_t n, int d, int form,
+                        float* D_device, double* ms) {
+  if (!ctx || !X_device || !D_device) return fail(SNN_EINVAL, "null argument");
+  if (n < 1 || d < 1) return fail(SNN_EINVAL, "bad sizes");
+  if (form != SNN_FORM_PLAIN && form != SNN_FORM_MFMA)
+    return fail(SNN_EINVAL, "form must be SNN_FORM_PLAIN or SNN_FORM_MFMA");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  HIPCHK(ctx->XT.ensure(sizeof(float) * (size_t)n * d));
+  if (form == SNN_FORM_MFMA) HIPCHK(ctx->scratch.ensure(snn::gram_mfma_scratch_bytes(n, d)));
+  Timing tm;
+  hipEvent_t e0 = tm.make(), e1 = tm.make();
+  HIPCHK(snn::launch_transpose(X_device, ctx->XT.as<float>(), n, d, st));
+  HIPCHK(hipEventRecord(e0, st));
+  if (form == SNN_FORM_MFMA)
+    HIPCHK(snn::launch_gram_mfma(X_device, ctx->XT.as<float>(), n, d, D_device, ctx->scratch.p,
+                                 st));
+  else
+    HIPCHK(snn::launch_gram_exact(ctx->XT.as<float>(), D_device, nullptr, nullptr, n, d, st));
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (ms) *ms = elapsed(e0, e1);
   return SNN_OK;
 }
 

@@ -14,7 +14,16 @@ hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const
 hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* Xc, int64_t n,
                           int d, int cpad, hipStream_t s);
 hipError_t launch_search_dense(const SearchProblem& a, hipStream_t s);
-hipError_t launch_search_pruned(const SearchProblem& a, hipStream_t s);
+// variant: 0 = by direction (forward: a wave per (query, candidate block);
+// backward: a workgroup per query sharing the row scan), 1 = per wave,
+// 2 = shared scan (where it applies: pruned_shared_applies).
+hipError_t launch_search_pruned(const SearchProblem& a, int variant, hipStream_t s);
+bool pruned_shared_applies(const SearchProblem& a);
+// BF16x3 MFMA Gram of the full-d distance matrix (gram.hip); scratch of
+// gram_mfma_scratch_bytes(n, d) bytes.
+size_t gram_mfma_scratch_bytes(int64_t n, int d);
+hipError_t launch_gram_mfma(const float* X, const float* XT, int64_t n, int d, float* D,
+                            void* scratch, hipStream_t s);
 // First forward step over sorted columns (a.skey, a.sidx, a.cand); grid over
 // max_cand candidate slots.  Returns at once where st->nonfinite is set.
 hipError_t launch_search_sorted(const SearchProblem& a, int max_cand, hipStream_t s);
@@ -50,6 +59,9 @@ hipError_t launch_init_state(SelState* st, const Problem& pb, int* cand, int* se
 // blocks of that kernel on the current device (call before graph capture).
 int64_t batch_dense_items(const SearchProblem& a);
 int64_t batch_pruned_items(const SearchProblem& a);
+// Seeds a backward pruned search reads for its bound (the nearest in D first):
+// 2K + 16 rounded up to 8, at most 128; SPEEDYNN_NSEED overrides it.
+int backward_seed_count(int kmax);
 int batch_grid(int mode, int mult, int K);
 hipError_t launch_batch_search(const SearchProblem* probs, const int64_t* start, int P,
                                unsigned long long* counter, int mode, int mult, int K, int grid,

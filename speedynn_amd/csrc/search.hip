@@ -3,6 +3,9 @@
 // candidate block) or one (query, candidate block) pair is a device function,
 // called by the single-problem kernels and by the persistent kernels that
 // serve many problems from one work queue.
+#include <cstdlib>
+#include <cstring>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -210,6 +213,49 @@ __device__ __forceinline__ void kth_insert(float (&B)[K], float v) {
   B[0] = v < B[0] ? v : B[0];
 }
 
+// Bound of one lane (candidate): the K-th smallest exact distance over the
+// query's seeds, which k_seed_scan lists by ascending D.  Forward distances
+// are >= D, so once the next seed's D exceeds every live lane's current K-th
+// value no later seed can lower a bound, and the scan stops; a backward
+// selection reads the first a.nseed seeds.  Returns +inf for an unfilled list,
+// -inf for padding lanes.
+template <int K, int MULT>
+__device__ __forceinline__ float seed_bound(const SearchProblem& a, const float* __restrict__ Xl,
+                                            const float* __restrict__ Dq, float xq, int64_t q,
+                                            int cpad, bool live) {
+  const float kInf = __int_as_float(0x7f800000);
+  float B[K];
+#pragma unroll
+  for (int m = 0; m < K; ++m) B[m] = kInf;
+  const int* __restrict__ sq = a.seeds + q * 128;
+  const int nseed = MULT > 0 ? 128 : a.nseed;
+  for (int s0 = 0; s0 < nseed; s0 += 8) {
+    int js[8];
+    float xs[8], dqs[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      js[u] = sq[s0 + u];
+      const int jj = js[u] < 0 ? 0 : js[u];
+      xs[u] = Xl[(int64_t)jj * cpad];
+      dqs[u] = Dq[jj];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (js[u] >= 0) {
+        const float t = xq - xs[u];
+        const float dd = MULT > 0 ? dqs[u] + t * t : dqs[u] - t * t;
+        if (dd < B[K - 1]) kth_insert<K>(B, dd);
+      }
+    }
+    if (MULT > 0 && s0 + 8 < nseed) {
+      const int jn = sq[s0 + 8];
+      const float bmax = wave_max(live ? B[K - 1] : -kInf);
+      if (jn < 0 || Dq[jn] > bmax) break;  // seeds ascend in D; missing ones come last
+    }
+  }
+  return live ? B[K - 1] : -kInf;
+}
+
 // Pruned search of one (query q, candidate block cb) item by one wave
 // (precision form: forward steps after the first, every backward step).
 //
@@ -258,32 +304,7 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* lab, 
   constexpr float kTiny = 1.0e-36f;
   auto dist = [](float dq, float p) { return MULT > 0 ? dq + p : dq - p; };
 
-  // bound: K-th smallest exact distance over the seeds
-  float B[K];
-#pragma unroll
-  for (int m = 0; m < K; ++m) B[m] = kInf;
-  const int* __restrict__ sq = a.seeds + q * 128;
-  for (int s0 = 0; s0 < 128; s0 += 8) {
-    int js[8];
-    float xs[8], dqs[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      js[u] = sq[s0 + u];
-      const int jj = js[u] < 0 ? 0 : js[u];
-      xs[u] = Xl[(int64_t)jj * cpad];
-      dqs[u] = Dq[jj];
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (js[u] >= 0) {
-        const float t = xq - xs[u];
-        const float p = t * t;
-        const float dd = dist(dqs[u], p);
-        if (dd < B[K - 1]) kth_insert<K>(B, dd);
-      }
-    }
-  }
-  float thr = live ? B[K - 1] : -kInf;
+  float thr = seed_bound<K, MULT>(a, Xl, Dq, xq, q, cpad, live);
   float T = wave_max(thr);
 
   float Ld[K];
@@ -495,6 +516,195 @@ __global__ __launch_bounds__(256) void k_search_pruned(const SearchProblem a, in
   pruned_wave<K, MULT>(a, lds + wave * (K * 64) + (threadIdx.x & 63), q, cb);
 }
 
+// Pruned search with the row scan shared by the candidate blocks of a query:
+// one workgroup per query, wave w owns candidate block w (waves past the
+// last block only help scanning).  Bounds and the exact path are those of
+// pruned_wave.  Per chunk of R*64*W points each lane tests R points against
+// the largest bound of the workgroup's waves; the passing points go, in
+// ascending index order, to a queue in LDS (index, D, lower bound), and every
+// wave visits the queue, skipping entries above its own bound.  The row is
+// read once per query instead of once per candidate block.  Bounds only
+// shrink, so a wave reading another wave's bound while it is being lowered
+// still gets a valid (larger) bound.  Barriers: counts written -> queue
+// written -> queue read (the next chunk's first barrier).
+template <int K, int MULT, int W>
+__global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchProblem a) {
+  constexpr int R = W >= 16 ? 2 : 4;  // points per lane per chunk (16 waves: 128 registers)
+  constexpr int SUB = 64 * W;
+  constexpr int CH = R * SUB;
+  constexpr int LDS_Q = 3 * CH;       // queue: index, D, lower bound
+  constexpr int LDS_E = 4 * K * 64;   // epilogue labels, 4 waves at a time
+  __shared__ float lds[LDS_Q > LDS_E ? LDS_Q : LDS_E];
+  __shared__ float Tw[W];
+  __shared__ int wcount[R * W];
+  int* qj = reinterpret_cast<int*>(lds);
+  float* qd = lds + CH;
+  float* qlb = lds + 2 * CH;
+
+  const SelState* st = a.st;
+  if (st->stopped) return;
+  const int ncand = st->ncand;
+  const int n = (int)a.n;  // n < 2^31 (checked on the host)
+  const int q = blockIdx.x;
+  const int wave = uniform(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int cb = wave;
+  const bool active = cb * 64 < ncand && cb < a.cpad / 64;
+  const int c = cb * 64 + lane;
+  const bool live = active && c < ncand;
+  const int cpad = a.cpad;
+  const int npf = (int)a.npf;
+  const int flo = (q / npf) * npf;
+  const int fspan = flo + npf < n ? npf : n - flo;  // fold of q: [flo, flo + fspan)
+  const float* __restrict__ Dq = a.D + (int64_t)q * n;
+  const float* __restrict__ Mq = MULT > 0 ? nullptr : a.M + (int64_t)q * n;
+  const float* __restrict__ Xl = a.Xc + (active ? cb * 64 : 0) + lane;
+  const float kInf = __int_as_float(0x7f800000);
+  constexpr float kScreen = 1.00000095367431640625f;  // 1 + 2^-20
+  constexpr float kTiny = 1.0e-36f;
+  auto dist = [](float dq, float p) { return MULT > 0 ? dq + p : dq - p; };
+  const float xq = active ? Xl[(int64_t)q * cpad] : 0.0f;
+
+  float thr = active ? seed_bound<K, MULT>(a, Xl, Dq, xq, q, cpad, live) : -kInf;
+  float T = wave_max(thr);
+  if (lane == 0) Tw[wave] = T;
+
+  float Ld[K];
+  int Li[K];
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    Ld[m] = kMaxF;
+    Li[m] = 0;
+  }
+  float lim = live ? kInf : -kInf;
+  unsigned long long seen = 0;
+
+  auto ldD = [&](int j) { return j < n ? Dq[j] : kInf; };
+  auto ldM = [&](int j) { return (MULT < 0 && j < n) ? Mq[j] : 0.0f; };
+  const int t0 = threadIdx.x;
+  float dn[R], mn[R];  // the next chunk's D (and M), loaded one chunk ahead
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    dn[r] = ldD(r * SUB + t0);
+    mn[r] = ldM(r * SUB + t0);
+  }
+  const uint32_t ucpad = (uint32_t)cpad;  // n * cpad < 2^31 (checked on the host)
+  __syncthreads();
+  for (int c0 = 0; c0 < n; c0 += CH) {
+    float Tmax = -kInf;
+#pragma unroll
+    for (int w = 0; w < W; ++w) Tmax = fmaxf(Tmax, Tw[w]);
+    float dv[R], lbv[R];
+    bool ok[R];
+    unsigned long long m[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int j = c0 + r * SUB + t0;
+      dv[r] = dn[r];
+      lbv[r] = MULT > 0 ? dn[r] : dn[r] - mn[r];
+      dn[r] = ldD(j + CH);
+      if (MULT < 0) mn[r] = ldM(j + CH);
+      ok[r] = j < n && (uint32_t)(j - flo) >= (uint32_t)fspan && lbv[r] <= Tmax;
+      m[r] = __ballot(ok[r]);
+      if (lane == 0) wcount[r * W + wave] = (int)__popcll(m[r]);
+    }
+    __syncthreads();
+    // queue order (sub-chunk, wave, lane) is ascending point index
+    int base[R], total = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      base[r] = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const int cw = wcount[r * W + w];
+        if (w == wave) base[r] = total;
+        total += cw;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (ok[r]) {
+        const int pos = base[r] + (int)__builtin_amdgcn_mbcnt_hi(
+                                      (uint32_t)(m[r] >> 32),
+                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m[r], 0u));
+        qj[pos] = c0 + r * SUB + t0;
+        qd[pos] = dv[r];
+        qlb[pos] = lbv[r];
+      }
+    }
+    __syncthreads();
+    if (!active) continue;
+    // the queue, 64 entries at a time: ballot of the entries within this
+    // wave's bound, then the visit loop of pruned_wave over its set bits
+    for (int e0 = 0; e0 < total; e0 += 64) {
+      const int e = e0 + lane;
+      const bool has = e < total;
+      const int je = has ? qj[e] : 0;
+      const float de = has ? qd[e] : 0.0f;
+      const float lbe = has ? qlb[e] : kInf;
+      unsigned long long mask = __ballot(has && lbe <= T);
+      if (!mask) continue;
+      seen += __popcll(mask);
+      bool changed = false;
+      while (mask) {
+        constexpr int BATCH = 8;
+        int js[BATCH];
+        float dqs[BATCH], ts[BATCH];
+        bool pass[BATCH];
+        int cnt = 0;
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) {
+          if (mask) {
+            const int b = __ffsll((long long)mask) - 1;
+            mask &= mask - 1;
+            js[u] = __builtin_amdgcn_readlane(je, b);
+            dqs[u] = lane_value(de, b);
+            ts[u] = Xl[(uint32_t)js[u] * ucpad];
+            cnt = u + 1;
+          }
+        }
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) {
+          ts[u] = xq - ts[u];
+          const float scr = MULT > 0 ? fmaf(ts[u], ts[u], dqs[u]) : dist(dqs[u], ts[u] * ts[u]);
+          pass[u] = u < cnt && scr < lim;
+          any |= pass[u];
+        }
+        if (__ballot(any)) {
+#pragma unroll
+          for (int u = 0; u < BATCH; ++u) {
+            if (pass[u]) {
+              const float dd = dist(dqs[u], ts[u] * ts[u]);
+              if (dd < Ld[K - 1]) {
+                topk_insert<K>(Ld, Li, dd, js[u]);
+                lim = MULT > 0 ? (Ld[K - 1] < kMaxF ? fmaf(Ld[K - 1], kScreen, kTiny) : kInf)
+                               : Ld[K - 1];
+                changed = true;
+              }
+            }
+          }
+        }
+      }
+      if (__ballot(changed)) {
+        thr = live ? fminf(thr, Ld[K - 1]) : -kInf;
+        T = wave_max(thr);
+        if (lane == 0) Tw[wave] = T;
+      }
+    }
+  }
+  if (a.visited && lane == 0 && active) {
+    const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
+    atomicAdd(a.visited, seen * (unsigned long long)nlive);
+  }
+  // epilogue, four waves at a time through the label area (aliases the queue)
+  for (int g = 0; g < W; g += 4) {
+    __syncthreads();
+    if (active && wave >= g && wave < g + 4)
+      finish_query<K>(a, Li, lds + (wave - g) * (K * 64) + lane, q, c, live);
+  }
+}
+
 // ------------------------------------------------------ persistent batch kernels
 //
 // Many independent problems per launch: a global counter hands out items
@@ -604,8 +814,23 @@ __global__ __launch_bounds__(256) void k_seed_scan(const float* __restrict__ D,
       }
     }
   }
-  seeds[q * 128 + lane] = i0;
-  seeds[q * 128 + 64 + lane] = i1;
+  // list the 128 seeds by ascending D (ties by slot): rank of each of the
+  // lane's two entries among all, on a total order of the float bits
+  auto sortable = [](float v) {
+    const unsigned int b = __float_as_uint(v);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  };
+  const unsigned int k0 = sortable(b0), k1 = sortable(b1);
+  int r0 = 0, r1 = 0;
+  for (int k = 0; k < 64; ++k) {
+    const unsigned int a0 = __shfl(k0, k), a1 = __shfl(k1, k);  // slots k and 64 + k
+    r0 += (a0 < k0 || (a0 == k0 && k < lane)) ? 1 : 0;
+    r0 += a1 < k0 ? 1 : 0;
+    r1 += a0 <= k1 ? 1 : 0;
+    r1 += (a1 < k1 || (a1 == k1 && k < lane)) ? 1 : 0;
+  }
+  seeds[q * 128 + r0] = i0;
+  seeds[q * 128 + r1] = i1;
 }
 
 // ============================================================ launchers
@@ -624,13 +849,63 @@ static hipError_t launch_dense_k(const SearchProblem& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Pruned-search variant: 0 = pick by shape, 1 = a wave per (query, candidate
+// block), 2 = a workgroup per query sharing the row scan.  Overridable with
+// SPEEDYNN_PRUNED=wave|shared for measurements.
+static int pruned_variant() {
+  static const int v = [] {
+    const char* e = std::getenv("SPEEDYNN_PRUNED");
+    if (!e) return 0;
+    if (!std::strcmp(e, "wave")) return 1;
+    if (!std::strcmp(e, "shared")) return 2;
+    return 0;
+  }();
+  return v;
+}
+
+int backward_seed_count(int kmax) {
+  static const int forced = [] {
+    const char* e = std::getenv("SPEEDYNN_NSEED");
+    return e ? std::atoi(e) : 0;
+  }();
+  int s = forced > 0 ? forced : 2 * search_list_size(kmax) + 16;
+  s = (s + 7) / 8 * 8;
+  return s < 8 ? 8 : (s > 128 ? 128 : s);
+}
+
+template <int K, int W>
+static void launch_pruned_shared(const SearchProblem& a, hipStream_t s) {
+  const dim3 g((unsigned)a.n), b(64 * W);
+  if (a.mult > 0) hipLaunchKernelGGL((k_search_pruned_shared<K, 1, W>), g, b, 0, s, a);
+  else hipLaunchKernelGGL((k_search_pruned_shared<K, -1, W>), g, b, 0, s, a);
+}
+
 template <int K>
-static hipError_t launch_pruned_k(const SearchProblem& a, hipStream_t s) {
+static hipError_t launch_pruned_k(const SearchProblem& a, int variant, hipStream_t s) {
+  const int nblk = a.cpad / 64;
+  // the shared-scan kernel: up to 16 candidate blocks (16 waves need lists of
+  // at most 20 to stay within 128 registers); its label area needs K <= 50
+  const int W = nblk <= 4 ? 4 : nblk <= 8 ? 8 : nblk <= 16 && K <= 20 ? 16 : 0;
+  // variant 0: forward takes the per-wave kernel (the candidate blocks'
+  // bounds differ widely when feature scales do, and a shared workgroup waits
+  // for its loosest block), backward the shared scan (two rows per query,
+  // similar bounds)
+  const int forced = pruned_variant();
+  const int v = forced ? forced : variant;
+  const bool shared = K <= 50 && W > 0 && (v == 2 || (v == 0 && a.mult < 0));
+  if (shared) {
+    if constexpr (K <= 50) {
+      if (W == 4) launch_pruned_shared<K, 4>(a, s);
+      else if (W == 8) launch_pruned_shared<K, 8>(a, s);
+      else if constexpr (K <= 20) launch_pruned_shared<K, 16>(a, s);
+    }
+    return hipGetLastError();
+  }
   // 4 independent waves per block; each XCD slice holds per_xcd*nblk waves.
   // forward: gather locality (candidate-block major); backward: the D and M
   // rows dominate (query major)
   const int64_t per_xcd = (a.n + 7) / 8;
-  const int64_t slice_blocks = (per_xcd * (a.cpad / 64) + 3) / 4;
+  const int64_t slice_blocks = (per_xcd * nblk + 3) / 4;
   const dim3 g((unsigned)(8 * slice_blocks)), b(256);
   if (a.mult > 0) hipLaunchKernelGGL((k_search_pruned<K, 1>), g, b, 0, s, a, 1);
   else hipLaunchKernelGGL((k_search_pruned<K, -1>), g, b, 0, s, a, 0);
@@ -668,14 +943,19 @@ hipError_t launch_search_sorted(const SearchProblem& a, int max_cand, hipStream_
   }
 }
 
-hipError_t launch_search_pruned(const SearchProblem& a, hipStream_t s) {
+hipError_t launch_search_pruned(const SearchProblem& a, int variant, hipStream_t s) {
   switch (search_list_size(a.kmax)) {
 #define SNN_CASE(K, QW, W) \
-  case K: return launch_pruned_k<K>(a, s);
+  case K: return launch_pruned_k<K>(a, variant, s);
     SNN_FOR_EACH_LIST_SIZE(SNN_CASE)
 #undef SNN_CASE
     default: return hipErrorInvalidValue;
   }
+}
+
+bool pruned_shared_applies(const SearchProblem& a) {
+  const int K = search_list_size(a.kmax), nblk = a.cpad / 64;
+  return K > 0 && K <= 50 && (nblk <= 8 || (nblk <= 16 && K <= 20)) && pruned_variant() == 0;
 }
 
 // Dense configuration (queries per wave, waves per block) of a list size.

@@ -186,3 +186,43 @@ def test_c1_full_size_matches_oracle(engine, form):
     steps = got["steps"]
     exp_c = np.array(rec["expected"]["candidate_errors_bits"], np.uint32).reshape(steps, -1)
     np.testing.assert_array_equal(got["candidate_errors"][:steps].view(np.uint32), exp_c)
+
+
+@pytest.mark.parametrize("name", ["reg_bwd_loo.json", "tiny_cls_fwd_loo.json"])
+def test_compensated_precision_selects_the_same(engine, name):
+    """Keeping D's compensation term (params.compensated) changes no bit of
+    the selection: the search reads D in the reference's bits either way."""
+    rec, X, y = load_case(os.path.join(GOLDEN, name))
+    inp = rec["inputs"]
+    codes = [LOSS_CODES[l] for l in inp["loss_functions"]]
+    print(f"compensated precision form on {name}", flush=True)
+    got = engine.select(X, y, inp["k_vals"], codes,
+                        method=1 if inp["method"] == "forward" else 0,
+                        model_type=1 if inp["model_type"] == "classification" else 0,
+                        k_fold=inp["k_fold"], num_features=inp["num_features"],
+                        auto_feature=inp["auto_feature"], form=1, compensated=True)
+    assert_same_result(got, expected_arrays(rec), name)
+
+
+def test_mfma_gram_within_1e5_of_fp64(engine):
+    """(a) on the matrix cores: the BF16x3 MFMA Gram (SNN_FORM_MFMA) against
+    the FP64 oracle, on unit-scale Gaussian features and on features with
+    scales 10^U(-2,2).  Tolerance 1e-5 relative; the diagonal is exactly 0."""
+    n, d = 1500, 300
+    Z = ds.gaussian_matrix(91, n, d)
+    scales = ds.exp2(np.log2(10.0) * (4.0 * ds.uniforms(92, 0, d) - 2.0))
+    cases = {"unit": Z, "scaled": (Z.astype(np.float64) * scales[None, :]).astype(np.float32)}
+    rows = [0, 7, 777, n - 1]
+    for name, X in cases.items():
+        print(f"mfma gram {name} n={n} d={d}", flush=True)
+        got, lo = engine.distance_rows(X, 0, [], rows, form=2)
+        assert lo is None
+        worst = 0.0
+        for r, q in enumerate(rows):
+            ref = O.distance_row_f64(X, q, list(range(d)))
+            off = np.arange(n) != q
+            rel = np.abs(got[r][off].astype(np.float64) - ref[off]) / ref[off]
+            worst = max(worst, float(rel.max()))
+            assert got[r][q] == 0.0
+        print(f"mfma gram {name}: max relative error {worst:.3e}", flush=True)
+        assert worst <= 1e-5, (name, worst)
