@@ -110,8 +110,12 @@ def cpu_baseline(X, y, kv, k_fold, model_type, method, target_s: float = 12.0) -
 
     nq = 16
     dt = run(nq)
-    nq = int(min(n, max(16, nq * target_s / max(dt, 1e-3))))
-    dt = run(nq)
+    while dt < 1.0 and nq < n:  # grow until the rate estimate is stable
+        nq = min(n, nq * 4)
+        dt = run(nq)
+    if dt < 0.5 * target_s and nq < n:
+        nq = int(min(n, nq * target_s / max(dt, 1e-3)))
+        dt = run(nq)
     step = "first forward step (D=0)" if mult > 0 else "first backward step (full D)"
     return {"value": nq * d / dt, "unit": "evals/s", "cores": threads, "kind": "port",
             "sample": f"CPU oracle, {step}: {nq} queries x {d} candidates x {n} points, "
@@ -215,7 +219,9 @@ def bench_single(args, hip, eng, form: int) -> dict:
                                   if r["evals"] else 0.0),
         "roofline": rf,
         "parity_vs_golden": parity,
-        "selected": r["selected_ordering"][:steps_sel].tolist(),
+        # forward: features in the order chosen; backward: the first 32 removed
+        "selected": (r["selected_ordering"][:steps_sel].tolist() if method == 1 else
+                     r["selected_ordering"][::-1][:min(steps_sel, 32)].tolist()),
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(X, y, w["k_vals"], w["k_fold"], mt, method)

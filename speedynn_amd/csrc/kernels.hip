@@ -444,24 +444,37 @@ __global__ __launch_bounds__(64) void k_loss_single(const float* __restrict__ y,
 
 // ------------------------------------------------------------- arg-min step
 //
-// One thread: best k per candidate (first k value, in k_vals order, with the
-// strictly smallest first-loss error; speedynn/lib/cpu.c:110-120), arg-min
-// over candidates in ascending feature order with strict '<'
-// (nearest_neighbors_feature_selection.c:318-329 forward, :155-167 backward),
-// records (:336-342 / :174-181), auto-stop (:446-465) and the next step's
-// candidate list.
-__global__ void k_select(SelState* __restrict__ st, const Problem pb,
-                         const int* __restrict__ kvals, const float* __restrict__ err,
-                         int* __restrict__ cand, int* __restrict__ sel, int* __restrict__ ordering,
-                         float* __restrict__ errors, float* __restrict__ val,
-                         int* __restrict__ bestk, float* __restrict__ cand_err,
-                         const int* __restrict__ perm) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One workgroup of 256 threads: best k per candidate (first k value, in
+// k_vals order, with the strictly smallest first-loss error;
+// speedynn/lib/cpu.c:110-120), arg-min over candidates in ascending feature
+// order with strict '<' (nearest_neighbors_feature_selection.c:318-329
+// forward, :155-167 backward), records (:336-342 / :174-181), auto-stop
+// (:446-465) and the next step's candidate list.  The reference's scan keeps
+// the first strictly smaller error, starting from FLT_MAX: the winner is the
+// least (error, feature) pair among errors below FLT_MAX, which a tree
+// reduction finds in any order.
+__global__ __launch_bounds__(256) void k_select(SelState* __restrict__ st, const Problem pb,
+                                                const int* __restrict__ kvals,
+                                                const float* __restrict__ err,
+                                                int* __restrict__ cand, int* __restrict__ sel,
+                                                int* __restrict__ ordering,
+                                                float* __restrict__ errors, float* __restrict__ val,
+                                                int* __restrict__ bestk,
+                                                float* __restrict__ cand_err,
+                                                const int* __restrict__ perm) {
+  __shared__ float s_v[256];
+  __shared__ int s_f[256], s_c[256];
+  __shared__ int s_count[4];
+  __shared__ int s_stop;
   if (st->stopped) return;
+  const int t = threadIdx.x;
   const int nc = st->ncand;
   const int cpad = pb.cpad, nk = pb.nk, nl = pb.nl, d = pb.d;
+  const int step = st->step;
   const float qnan = __int_as_float(0x7fc00000);
-  for (int c = 0; c < nc; ++c) {
+  float bv = kMaxF;
+  int bf = 0x7fffffff, bc = -1;
+  for (int c = t; c < nc; c += 256) {
     float smallest = kMaxF;
     int bm = -1;
     for (int m = 0; m < nk; ++m) {
@@ -474,53 +487,92 @@ __global__ void k_select(SelState* __restrict__ st, const Problem pb,
     for (int l = 0; l < nl; ++l)
       val[l * cpad + c] = bm >= 0 ? err[((int64_t)l * nk + bm) * cpad + c] : qnan;
     bestk[c] = bm >= 0 ? kvals[bm] : 0;
-    if (cand_err) cand_err[(int64_t)st->step * d + cand[c]] = val[c];
-  }
-  // slots may be in any feature order: the smallest error wins, equal errors go
-  // to the lower feature index -- the outcome of the reference's ascending scan
-  float minv = kMaxF;
-  int best = -1;
-  for (int c = 0; c < nc; ++c) {
-    const float v = val[c];
-    if (v < minv || (best >= 0 && v == minv && cand[c] < cand[best])) {
-      minv = v;
-      best = c;
+    const float v = bm >= 0 ? err[(int64_t)bm * cpad + c] : qnan;
+    const int f = cand[c];
+    if (cand_err) cand_err[(int64_t)step * d + f] = v;
+    if (v < kMaxF && (v < bv || (v == bv && f < bf))) {
+      bv = v;
+      bf = f;
+      bc = c;
     }
   }
+  s_v[t] = bv;
+  s_f[t] = bf;
+  s_c[t] = bc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) {
+      const float v2 = s_v[t + o];
+      const int f2 = s_f[t + o];
+      if (s_c[t + o] >= 0 && (s_c[t] < 0 || v2 < s_v[t] || (v2 == s_v[t] && f2 < s_f[t]))) {
+        s_v[t] = v2;
+        s_f[t] = f2;
+        s_c[t] = s_c[t + o];
+      }
+    }
+    __syncthreads();
+  }
+  const int best = s_c[0];
   if (best < 0) {
-    st->error = 1;
-    st->stopped = 2;
+    if (t == 0) {
+      st->error = 1;
+      st->stopped = 2;
+    }
     return;
   }
-  const int f = cand[best];
-  const int pos = st->pos;
   const bool fwd = pb.method == 1;
-  sel[f] = fwd ? 1 : 0;
-  ordering[pos] = f;
-  for (int l = 0; l < nl; ++l) errors[l * d + pos] = val[l * cpad + best];
-  st->best_k = bestk[best];
-  st->fstar = f;
-  st->fslot = best;
-  st->pending = 1;
-  st->num_final += fwd ? 1 : -1;
-  st->step += 1;
-  bool stop = false;
-  if (pb.auto_feature) {
-    if (fwd) stop = pos > 0 && errors[pos - 1] < errors[pos];
-    else stop = pos > 0 && pos < d - 1 && errors[pos + 1] < errors[pos];
+  if (t == 0) {
+    const int f = cand[best];
+    const int pos = st->pos;
+    sel[f] = fwd ? 1 : 0;
+    ordering[pos] = f;
+    for (int l = 0; l < nl; ++l) errors[l * d + pos] = val[l * cpad + best];
+    st->best_k = bestk[best];
+    st->fstar = f;
+    st->fslot = best;
+    st->pending = 1;
+    st->num_final += fwd ? 1 : -1;
+    st->step = step + 1;
+    bool stop = false;
+    if (pb.auto_feature) {
+      if (fwd) stop = pos > 0 && errors[pos - 1] < errors[pos];
+      else stop = pos > 0 && pos < d - 1 && errors[pos + 1] < errors[pos];
+    }
+    const int next = fwd ? pos + 1 : pos - 1;
+    if (fwd ? !(next < pb.num_desired) : !(next >= pb.num_desired)) stop = true;
+    st->pos = next;
+    s_stop = stop ? 1 : 0;
   }
-  const int next = fwd ? pos + 1 : pos - 1;
-  if (fwd ? !(next < pb.num_desired) : !(next >= pb.num_desired)) stop = true;
-  st->pos = next;
+  __syncthreads();  // sel[f] written; every read of cand[] above is done
+  // next candidate list: the unselected (forward) or selected (backward)
+  // features in slot order, compacted with a ballot per wave
   const int check = fwd ? 0 : 1;
+  const int wave = t >> 6;
   int nc2 = 0;
-  for (int gi = 0; gi < d; ++gi) {
-    const int g = perm[gi];
-    if (sel[g] == check) cand[nc2++] = g;
+  for (int base = 0; base < d; base += 256) {
+    const int gi = base + t;
+    const int g = gi < d ? perm[gi] : 0;
+    const bool keep = gi < d && sel[g] == check;
+    const unsigned long long m = __ballot(keep);
+    if ((t & 63) == 0) s_count[wave] = (int)__popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      off += w < wave ? s_count[w] : 0;
+      tot += s_count[w];
+    }
+    if (keep)
+      cand[nc2 + off +
+           (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = g;
+    nc2 += tot;
+    __syncthreads();
   }
-  st->ncand = nc2;
-  if (nc2 == 0) stop = true;
-  st->stopped = stop ? 1 : 0;
+  if (t == 0) {
+    st->ncand = nc2;
+    st->stopped = (s_stop || nc2 == 0) ? 1 : 0;
+  }
 }
 
 // Candidate slot order.  The pruned search bounds a block of 64 candidates by
@@ -676,7 +728,7 @@ bool needs_median_scratch(const int* codes, int count) {
 hipError_t launch_select(SelState* st, const Problem& pb, const int* kvals, const float* err,
                          int* cand, int* sel, int* ordering, float* errors, float* val,
                          int* bestk, float* cand_err, const int* perm, hipStream_t s) {
-  hipLaunchKernelGGL(k_select, dim3(1), dim3(64), 0, s, st, pb, kvals, err, cand, sel, ordering,
+  hipLaunchKernelGGL(k_select, dim3(1), dim3(256), 0, s, st, pb, kvals, err, cand, sel, ordering,
                      errors, val, bestk, cand_err, perm);
   return hipGetLastError();
 }

@@ -799,9 +799,8 @@ __global__ __launch_bounds__(256) void k_seed_scan(const float* __restrict__ D,
   const float kInf = __int_as_float(0x7f800000);
   float b0 = kInf, b1 = kInf;
   int i0 = -1, i1 = -1;
-  for (int64_t j = lane; j < n; j += 64) {
-    if (j >= flo && j < fhi) continue;
-    const float v = Dq[j];
+  auto offer = [&](float v, int64_t j) {
+    if (j >= flo && j < fhi) return;
     if (v < b1) {
       if (v < b0) {
         b1 = b0;
@@ -813,7 +812,16 @@ __global__ __launch_bounds__(256) void k_seed_scan(const float* __restrict__ D,
         i1 = (int)j;
       }
     }
+  };
+  int64_t j = lane;
+  for (; j + 7 * 64 < n; j += 8 * 64) {  // eight loads in flight per lane
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = Dq[j + u * 64];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) offer(v[u], j + u * 64);
   }
+  for (; j < n; j += 64) offer(Dq[j], j);
   // list the 128 seeds by ascending D (ties by slot): rank of each of the
   // lane's two entries among all, on a total order of the float bits
   auto sortable = [](float v) {

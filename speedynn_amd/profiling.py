@@ -55,10 +55,13 @@ def pmc_means(path: str, kernel: str | None = None) -> dict:
 
 
 def _one(means: dict, counter: str) -> tuple[str, float, int]:
+    """Mean per dispatch over every kernel the pattern matched (dispatch-weighted)."""
     hits = [(k[0], v[0], v[1]) for k, v in means.items() if k[1] == counter]
-    if len(hits) != 1:
-        raise SystemExit(f"expected one kernel with {counter}, found {[h[0] for h in hits]}")
-    return hits[0]
+    if not hits:
+        raise SystemExit(f"no kernel with {counter}")
+    total = sum(m * c for _, m, c in hits)
+    count = sum(c for _, _, c in hits)
+    return " + ".join(sorted(h[0] for h in hits)), total / count, count
 
 
 def traffic_markdown(a) -> str:
