@@ -118,14 +118,15 @@ double elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 // Autotuner of the pruned search.  Which kernel is faster -- a wave per
-// (query, candidate block), or a workgroup per query sharing the row scan --
-// depends on the data (how much the candidate blocks' bounds differ) and on
-// the step (early steps keep many points, late ones few), not only on the
-// shape.  The first step of a shape (n, d, list size, direction) and step
-// range [2^b, 2^(b+1)) in this process runs both on the same step, timed
-// with HIP events (their results are bit-identical; the first run does not
-// count visits), and the faster serves the rest of that range.
-// SPEEDYNN_TUNING=off keeps the rule by direction.
+// (query, candidate block), a workgroup per query sharing the row scan, or
+// that scan split over two workgroups per query -- depends on the data (how
+// much the candidate blocks' bounds differ) and on the step (early steps keep
+// many points, late ones few), not only on the shape.  The first step of a
+// shape (n, d, list size, direction) and step range [2^b, 2^(b+1)) in this
+// process runs every applicable kernel on the same step, timed with HIP
+// events (their results are bit-identical; only the last run counts visits),
+// and the fastest serves the rest of that range.  SPEEDYNN_TUNING=off keeps
+// the rule by direction.
 std::mutex g_tune_mu;
 std::map<std::tuple<int64_t, int, int, int, int>, int> g_tuned;
 
@@ -138,9 +139,12 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, hipStream
   int bucket = 0;
   while ((2 << bucket) <= step + 1) ++bucket;
   const auto key = std::make_tuple(sl.n, d, snn::search_list_size(sl.kmax), sl.mult, bucket);
+  int cands[3] = {1, 0, 0}, nc = 1;  // the kernels that apply to this shape
+  if (snn::pruned_shared_applies(sl)) cands[nc++] = 2;
+  if (snn::pruned_split_applies(sl)) cands[nc++] = 3;
   int variant = 0;
   bool tune = false;
-  if (!off && snn::pruned_shared_applies(sl)) {
+  if (!off && nc > 1) {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tuned.find(key);
     if (it != g_tuned.end()) variant = it->second;
@@ -152,22 +156,32 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, hipStream
     *used |= 1 << (v - 1);
     return SNN_OK;
   }
+  // every applicable kernel on this step; only the last run counts visits
   Timing tm;
-  hipEvent_t t0 = tm.make(), t1 = tm.make(), t2 = tm.make();
+  hipEvent_t ev[4];
+  for (int i = 0; i <= nc; ++i) ev[i] = tm.make();
   snn::SearchProblem quiet = sl;
   quiet.visited = nullptr;
-  HIPCHK(hipEventRecord(t0, s));
-  HIPCHK(snn::launch_search_pruned(quiet, 1, s));
-  HIPCHK(hipEventRecord(t1, s));
-  HIPCHK(snn::launch_search_pruned(sl, 2, s));
-  HIPCHK(hipEventRecord(t2, s));
-  HIPCHK(hipEventSynchronize(t2));
-  variant = elapsed(t1, t2) < elapsed(t0, t1) ? 2 : 1;
+  HIPCHK(hipEventRecord(ev[0], s));
+  for (int i = 0; i < nc; ++i) {
+    HIPCHK(snn::launch_search_pruned(i + 1 < nc ? quiet : sl, cands[i], s));
+    HIPCHK(hipEventRecord(ev[i + 1], s));
+    *used |= 1 << (cands[i] - 1);
+  }
+  HIPCHK(hipEventSynchronize(ev[nc]));
+  variant = cands[0];
+  double best = elapsed(ev[0], ev[1]);
+  for (int i = 1; i < nc; ++i) {
+    const double t = elapsed(ev[i], ev[i + 1]);
+    if (t < best) {
+      best = t;
+      variant = cands[i];
+    }
+  }
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     g_tuned[key] = variant;
   }
-  *used |= 3;  // this step ran both
   return SNN_OK;
 }
 

@@ -16,9 +16,11 @@ hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* 
 hipError_t launch_search_dense(const SearchProblem& a, hipStream_t s);
 // variant: 0 = by direction (forward: a wave per (query, candidate block);
 // backward: a workgroup per query sharing the row scan), 1 = per wave,
-// 2 = shared scan (where it applies: pruned_shared_applies).
+// 2 = shared scan (where pruned_shared_applies), 3 = shared scan split over
+// two workgroups per query (where pruned_split_applies).
 hipError_t launch_search_pruned(const SearchProblem& a, int variant, hipStream_t s);
 bool pruned_shared_applies(const SearchProblem& a);
+bool pruned_split_applies(const SearchProblem& a);
 // BF16x3 MFMA Gram of the full-d distance matrix (gram.hip); scratch of
 // gram_mfma_scratch_bytes(n, d) bytes.
 size_t gram_mfma_scratch_bytes(int64_t n, int d);

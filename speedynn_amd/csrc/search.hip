@@ -548,7 +548,7 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
   const int q = blockIdx.x;
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int cb = wave;
+  const int cb = wave + W * blockIdx.y;  // several workgroups per query split the blocks
   const bool active = cb * 64 < ncand && cb < a.cpad / 64;
   const int c = cb * 64 + lane;
   const bool live = active && c < ncand;
@@ -883,7 +883,8 @@ int backward_seed_count(int kmax) {
 
 template <int K, int W>
 static void launch_pruned_shared(const SearchProblem& a, hipStream_t s) {
-  const dim3 g((unsigned)a.n), b(64 * W);
+  // workgroups per query: enough W-wave groups to cover the candidate blocks
+  const dim3 g((unsigned)a.n, (unsigned)((a.cpad / 64 + W - 1) / W)), b(64 * W);
   if (a.mult > 0) hipLaunchKernelGGL((k_search_pruned_shared<K, 1, W>), g, b, 0, s, a);
   else hipLaunchKernelGGL((k_search_pruned_shared<K, -1, W>), g, b, 0, s, a);
 }
@@ -906,6 +907,16 @@ static hipError_t launch_pruned_k(const SearchProblem& a, int variant, hipStream
       if (W == 4) launch_pruned_shared<K, 4>(a, s);
       else if (W == 8) launch_pruned_shared<K, 8>(a, s);
       else if constexpr (K <= 20) launch_pruned_shared<K, 16>(a, s);
+    }
+    return hipGetLastError();
+  }
+  // variant 3: the shared scan split over two workgroups per query (half the
+  // candidate blocks each; the row is read twice, but two workgroups share a
+  // CU, so one's barrier waits hide behind the other's work)
+  if (v == 3 && K <= 50 && nblk > 4 && nblk <= 16) {
+    if constexpr (K <= 50) {
+      if (nblk <= 8) launch_pruned_shared<K, 4>(a, s);
+      else launch_pruned_shared<K, 8>(a, s);
     }
     return hipGetLastError();
   }
@@ -964,6 +975,11 @@ hipError_t launch_search_pruned(const SearchProblem& a, int variant, hipStream_t
 bool pruned_shared_applies(const SearchProblem& a) {
   const int K = search_list_size(a.kmax), nblk = a.cpad / 64;
   return K > 0 && K <= 50 && (nblk <= 8 || (nblk <= 16 && K <= 20)) && pruned_variant() == 0;
+}
+
+bool pruned_split_applies(const SearchProblem& a) {
+  const int K = search_list_size(a.kmax), nblk = a.cpad / 64;
+  return K > 0 && K <= 50 && nblk > 4 && nblk <= 16 && pruned_variant() == 0;
 }
 
 // Dense configuration (queries per wave, waves per block) of a list size.
