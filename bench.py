@@ -175,6 +175,8 @@ def bench_single(args, hip, eng, form: int) -> dict:
             names.append(f"k_search_pruned_shared<{K},{mult},{W}>")
         if r["pruned_kernel"] & 4:  # the same kernel, two workgroups per query
             names.append(f"k_search_pruned_shared<{K},{mult},{4 if nblk <= 8 else 8}> x2")
+        if r["pruned_kernel"] & 8:  # the second step's walk along a sorted column
+            names.append(f"k_search_walk<{K}>")
         kernel = " + ".join(names) or f"k_search_pruned<{K},{mult}>"
         first = 1 if method == 1 else 0
         ms_dom = sum(x["ms_search"] - (x["ms_search_first"] if first else 0.0) for x in results)

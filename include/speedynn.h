@@ -105,7 +105,8 @@ typedef struct {
     int steps;           /* selection steps executed */
     int pruned_kernel;   /* pruned searches used, bit mask: 1 a wave per (query,
                             candidate block), 2 a workgroup per query sharing the row
-                            scan, 4 that scan split over two workgroups per query */
+                            scan, 4 that scan split over two workgroups per query,
+                            8 the second forward step's walk along a sorted column */
     double ms_total;     /* device time of the selection (HIP events) */
     double ms_distance;  /* (a) initial matrix */
     double ms_search;    /* (c) candidate search kernels only */

@@ -34,6 +34,7 @@ struct Slot {
   size_t X, y, XT, D, Dlo, M, Xc, P, err, val, bestk, cand, sel, ordering, errors, kvals, losses,
       state, perm, key, seeds, med, skey, sidx, sort_tmp;
   bool has_med;      // median losses: sort space for NaN-holding error vectors
+  int loss_paths;    // which loss kernels the problem's losses need
   bool sorted;       // forward, precision form: sorted columns for the first step
   size_t sort_bytes;
   bool comp;         // keep D's compensation term
@@ -143,7 +144,7 @@ struct Plan {
       SelState* st = at<SelState>(A(), sl.state);
       HIPCHK(snn::launch_loss(at<float>(A(), sl.P), sl.dy, at<int>(A(), sl.losses),
                               st, at<float>(A(), sl.err), pb.n, pb.cpad, pb.nk, pb.nl, pb.d,
-                              sl.has_med ? at<float>(A(), sl.med) : nullptr, s));
+                              sl.has_med ? at<float>(A(), sl.med) : nullptr, sl.loss_paths, s));
       HIPCHK(snn::launch_select(st, pb, at<int>(A(), sl.kvals), at<float>(A(), sl.err),
                                 at<int>(A(), sl.cand), at<int>(A(), sl.sel),
                                 at<int>(A(), sl.ordering), at<float>(A(), sl.errors),
@@ -270,6 +271,7 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     s.seeds = pruned ? take(off, sizeof(int) * n * 128) : 0;
     const snn_problem& q = probs[s.index];
     s.has_med = snn::needs_median_scratch(q.loss_functions, q.num_loss);
+    s.loss_paths = snn::loss_paths(q.loss_functions, q.num_loss);
     s.med = s.has_med ? take(off, sizeof(float) * 2 * n * cpad * pb.nk) : 0;
     s.sorted = pruned && pb.method == 1;
     s.sort_bytes = s.sorted ? snn::sort_columns_temp_bytes(pb.n, pb.d) : 0;
@@ -416,7 +418,8 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
       const Problem& pb = sl.pb;
       const int64_t f[] = {sl.index, pb.n, pb.npf, pb.d, pb.nk, pb.nl, pb.kmax, pb.method,
                            pb.model_type, pb.auto_feature, pb.max_steps, pb.num_desired,
-                           sl.comp, sl.has_med, (int64_t)(size_t)sl.dX, (int64_t)(size_t)sl.dy};
+                           sl.comp, sl.has_med, sl.loss_paths, (int64_t)(size_t)sl.dX,
+                           (int64_t)(size_t)sl.dy};
       for (int64_t v : f) key.push_back((size_t)v);
     }
     if (key != cache.key || old_arena != cache.arena.p || old_meta != cache.meta.p ||

@@ -69,6 +69,7 @@ struct snn_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   snn::DevBuf X, y, XT, D, Dlo, Xc, P, err, val, bestk, cand, sel, ordering, errors, cand_err,
-      kvals, losses, state, scratch, seeds, visited, perm, key, M, med, skey, sidx, sort_tmp;
+      kvals, losses, state, scratch, seeds, visited, perm, key, M, med, skey, sidx, sort_tmp,
+      rank;
   snn::BatchCache batch;
 };

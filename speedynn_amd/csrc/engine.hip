@@ -130,18 +130,28 @@ double elapsed(hipEvent_t a, hipEvent_t b) {
 std::mutex g_tune_mu;
 std::map<std::tuple<int64_t, int, int, int, int>, int> g_tuned;
 
-int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, hipStream_t s,
-                        int* used) {
+// Variant 4: the walk along the first chosen feature's sorted column, for a
+// forward selection's second step (rank != null); it has a step range of its own.
+hipError_t launch_variant(const snn::SearchProblem& sl, int variant, const int* rank,
+                          hipStream_t s) {
+  if (variant == 4) return snn::launch_search_walk(sl, rank, s);
+  return snn::launch_search_pruned(sl, variant, s);
+}
+
+int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int* rank,
+                        hipStream_t s, int* used) {
   static const bool off = [] {
     const char* e = std::getenv("SPEEDYNN_TUNING");
     return e && !std::strcmp(e, "off");
   }();
   int bucket = 0;
   while ((2 << bucket) <= step + 1) ++bucket;
+  if (rank) bucket = 64;
   const auto key = std::make_tuple(sl.n, d, snn::search_list_size(sl.kmax), sl.mult, bucket);
-  int cands[3] = {1, 0, 0}, nc = 1;  // the kernels that apply to this shape
+  int cands[4] = {1, 0, 0, 0}, nc = 1;  // the kernels that apply to this shape
   if (snn::pruned_shared_applies(sl)) cands[nc++] = 2;
   if (snn::pruned_split_applies(sl)) cands[nc++] = 3;
+  if (rank) cands[nc++] = 4;
   int variant = 0;
   bool tune = false;
   if (!off && nc > 1) {
@@ -151,20 +161,20 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, hipStream
     else tune = true;
   }
   if (!tune) {
-    HIPCHK(snn::launch_search_pruned(sl, variant, s));
+    HIPCHK(launch_variant(sl, variant, rank, s));
     const int v = variant ? variant : (sl.mult > 0 || !snn::pruned_shared_applies(sl) ? 1 : 2);
     *used |= 1 << (v - 1);
     return SNN_OK;
   }
   // every applicable kernel on this step; only the last run counts visits
   Timing tm;
-  hipEvent_t ev[4];
+  hipEvent_t ev[5];
   for (int i = 0; i <= nc; ++i) ev[i] = tm.make();
   snn::SearchProblem quiet = sl;
   quiet.visited = nullptr;
   HIPCHK(hipEventRecord(ev[0], s));
   for (int i = 0; i < nc; ++i) {
-    HIPCHK(snn::launch_search_pruned(i + 1 < nc ? quiet : sl, cands[i], s));
+    HIPCHK(launch_variant(i + 1 < nc ? quiet : sl, cands[i], rank, s));
     HIPCHK(hipEventRecord(ev[i + 1], s));
     *used |= 1 << (cands[i] - 1);
   }
@@ -237,6 +247,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     sort_bytes = snn::sort_columns_temp_bytes(n, d);
     if (!sort_bytes) return fail(SNN_EDEVICE, "segmented sort: no temporary storage size");
     HIPCHK(ctx->sort_tmp.ensure(sort_bytes));
+    HIPCHK(ctx->rank.ensure(sizeof(int) * (size_t)n));
   }
   float* med = nullptr;  // median losses: sort space for NaN-holding error vectors
   if (snn::needs_median_scratch(loss_functions, num_loss)) {
@@ -324,7 +335,13 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     if (pruned_step) HIPCHK(snn::launch_seed_scan(D, st, ctx->seeds.as<int>(), n, pb.npf, s));
     HIPCHK(hipEventRecord(b2, s));
     if (pruned_step) {
-      int rc2 = search_pruned_tuned(sl, d, step, s, &used_variant);
+      // the second forward step may walk the first chosen feature's sorted column
+      const int* rank = nullptr;
+      if (sorted_first && step == 1) {
+        HIPCHK(snn::launch_rank(ctx->sidx.as<int>(), st, ctx->rank.as<int>(), n, s));
+        rank = ctx->rank.as<int>();
+      }
+      int rc2 = search_pruned_tuned(sl, d, step, rank, s, &used_variant);
       if (rc2) return rc2;
     } else {
       // first forward step (D = 0): sorted columns; the dense search runs
@@ -335,7 +352,8 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     }
     HIPCHK(hipEventRecord(c, s));
     HIPCHK(snn::launch_loss(ctx->P.as<float>(), dy, ctx->losses.as<int>(), st,
-                            ctx->err.as<float>(), n, pb.cpad, num_k, num_loss, d, med, s));
+                            ctx->err.as<float>(), n, pb.cpad, num_k, num_loss, d, med,
+                            snn::loss_paths(loss_functions, num_loss), s));
     HIPCHK(snn::launch_select(st, pb, ctx->kvals.as<int>(), ctx->err.as<float>(),
                               ctx->cand.as<int>(), ctx->sel.as<int>(), ctx->ordering.as<int>(),
                               ctx->errors.as<float>(), ctx->val.as<float>(), ctx->bestk.as<int>(),
@@ -479,7 +497,7 @@ void snn_destroy(snn_ctx* c) {
                     &c->bestk, &c->cand, &c->sel, &c->ordering, &c->errors, &c->cand_err,
                     &c->kvals, &c->losses, &c->state, &c->scratch, &c->seeds,
                     &c->visited, &c->perm, &c->key, &c->M, &c->med, &c->skey,
-                    &c->sidx, &c->sort_tmp};
+                    &c->sidx, &c->sort_tmp, &c->rank};
   for (DevBuf* b : bufs) b->release();
   c->batch.drop_graphs();
   c->batch.arena.release();

@@ -21,6 +21,11 @@ hipError_t launch_search_dense(const SearchProblem& a, hipStream_t s);
 hipError_t launch_search_pruned(const SearchProblem& a, int variant, hipStream_t s);
 bool pruned_shared_applies(const SearchProblem& a);
 bool pruned_split_applies(const SearchProblem& a);
+// Second forward step of the precision form: the walk along the first chosen
+// feature's sorted column (a.skey, a.sidx; rank from launch_rank after the
+// first step's arg-min).
+hipError_t launch_search_walk(const SearchProblem& a, const int* rank, hipStream_t s);
+hipError_t launch_rank(const int* sidx, const SelState* st, int* rank, int64_t n, hipStream_t s);
 // BF16x3 MFMA Gram of the full-d distance matrix (gram.hip); scratch of
 // gram_mfma_scratch_bytes(n, d) bytes.
 size_t gram_mfma_scratch_bytes(int64_t n, int d);
@@ -40,9 +45,13 @@ hipError_t launch_seed_scan(const float* D, const SelState* st, int* seeds, int6
 // scratch: median losses with NaN errors sort a copy there, 2n floats per
 // (k value, candidate slot) -- nk * cpad * 2n for launch_loss, 2n for
 // launch_loss_single; may be null when needs_median_scratch() is false.
+// paths = loss_paths(host copy of the loss codes): bit 1 the wave-per-candidate
+// kernel (zero_one, medians, composites), bit 2 the lane-per-candidate kernel
+// (the sequential sums: rmse, rmse_norm, std, rmse_std).
+int loss_paths(const int* codes, int count);
 hipError_t launch_loss(const float* P, const float* y, const int* losses, const SelState* st,
                        float* err, int64_t n, int cpad, int nk, int nl, int max_cand,
-                       float* scratch, hipStream_t s);
+                       float* scratch, int paths, hipStream_t s);
 hipError_t launch_loss_single(const float* y, const float* p, int64_t n, int code, float* out,
                               float* scratch, hipStream_t s);
 bool needs_median_scratch(const int* codes, int count);

@@ -229,25 +229,33 @@ __global__ __launch_bounds__(256) void k_compact(const float* __restrict__ X,
 // ------------------------------------------------------------------ losses
 
 // Sequential float sum over q = 0..n-1 (or n-1..0) of per-point terms, in
-// exactly that order, one wave: lanes compute 64 terms, then the terms are
-// added one by one (readlane) so the rounding matches a scalar loop.
+// exactly that order, one wave: lanes compute 64 terms at a time, which are
+// added one by one (readlane) so the rounding matches a scalar loop.  The
+// next 256 terms are loaded while the current 256 are added.
 template <bool REV, class Term>
 __device__ float seq_sum(int64_t n, Term term) {
   const int lane = threadIdx.x & 63;
-  float s = 0.0f;
-  if (!REV) {
-    for (int64_t q0 = 0; q0 < n; q0 += 64) {
-      const int64_t q = q0 + lane;
-      const float t = q < n ? term(q) : 0.0f;
-      const int cnt = (int)((n - q0) < 64 ? (n - q0) : 64);
-      for (int i = 0; i < cnt; ++i) s = s + lane_value(t, i);
+  constexpr int U = 4;  // 64*U terms per batch
+  auto fetch = [&](int64_t k0, float (&t)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = k0 + u * 64 + lane;  // k-th term in summation order
+      t[u] = k < n ? term(REV ? n - 1 - k : k) : 0.0f;
     }
-  } else {
-    for (int64_t q1 = n; q1 > 0; q1 -= 64) {
-      const int64_t base = q1 - 64 > 0 ? q1 - 64 : 0;
-      const int64_t q = base + lane;
-      const float t = q < q1 ? term(q) : 0.0f;
-      for (int i = (int)(q1 - base) - 1; i >= 0; --i) s = s + lane_value(t, i);
+  };
+  float s = 0.0f;
+  float cur[U], nxt[U];
+  fetch(0, nxt);
+  for (int64_t k0 = 0; k0 < n; k0 += 64 * U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) cur[u] = nxt[u];
+    if (k0 + 64 * U < n) fetch(k0 + 64 * U, nxt);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t kb = k0 + u * 64;
+      if (kb >= n) break;
+      const int cnt = (int)((n - kb) < 64 ? (n - kb) : 64);
+      for (int i = 0; i < cnt; ++i) s = s + lane_value(cur[u], i);
     }
   }
   return s;
@@ -280,6 +288,115 @@ __device__ float loss_rmse_norm(const float* __restrict__ y, const float* __rest
     return t * t;
   });
   return (float)sqrt((double)(s / (float)n));
+}
+
+// The same sums with one candidate per lane: lane c adds its own terms in the
+// reference's order, so one VALU add serves 64 candidates and no term goes
+// through readlane.  The 64 candidates' rows are processed in tiles of 64
+// points: kLossLoaders loader waves read the tile's rows coalesced (256 bytes
+// per row), evaluate the terms and store them transposed in LDS; the last
+// wave of the block only adds, lane c along row c of the tile, while the
+// loaders work on the next tile (double-buffered, one barrier per tile).
+// term(q, p_q) gives the addend of point q from the prediction p_q.
+constexpr int kLossTile = 64;
+constexpr int kLossLoaders = 8;
+constexpr int kLossRows = 64 / kLossLoaders;   // rows per loader wave
+constexpr int kLossStride = kLossTile + 4;     // 16-byte aligned rows, distinct banks
+constexpr int kLossThreads = (kLossLoaders + 1) * 64;
+constexpr int kLossLdsFloats = 2 * 64 * kLossStride;
+
+template <bool REV, class Term>
+__device__ __forceinline__ float lane_seq_sum(const float* __restrict__ P0, int64_t n,
+                                              float* __restrict__ tiles, Term term) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t ntile = (n + kLossTile - 1) / kLossTile;
+  float s = 0.0f;
+  float raw[kLossRows];
+  auto fetch = [&](int64_t t) {  // loader: its rows of tile t
+    const int64_t k = t * kLossTile + lane;  // k-th point in summation order
+    const int64_t q = REV ? n - 1 - k : k;
+    const float* base = P0 + (int64_t)(wave * kLossRows) * n + q;
+#pragma unroll
+    for (int r = 0; r < kLossRows; ++r) raw[r] = k < n ? base[(int64_t)r * n] : 0.0f;
+  };
+  auto stage = [&](int64_t t, float* buf) {
+    const int64_t k = t * kLossTile + lane;
+    const int64_t q = REV ? n - 1 - k : k;
+#pragma unroll
+    for (int r = 0; r < kLossRows; ++r)
+      buf[(wave * kLossRows + r) * kLossStride + lane] = k < n ? term(q, raw[r]) : 0.0f;
+  };
+  if (wave < kLossLoaders) {
+    fetch(0);
+    stage(0, tiles);
+    if (ntile > 1) fetch(1);
+  }
+  __syncthreads();
+  for (int64_t t = 0; t < ntile; ++t) {
+    const float* cur = tiles + (t & 1) * 64 * kLossStride;
+    if (wave == kLossLoaders) {
+      const float* row = cur + lane * kLossStride;
+      if (n - t * kLossTile >= kLossTile) {
+#pragma unroll
+        for (int i = 0; i < kLossTile; i += 4) {
+          const float4 v = *reinterpret_cast<const float4*>(row + i);
+          s = s + v.x;
+          s = s + v.y;
+          s = s + v.z;
+          s = s + v.w;
+        }
+      } else {
+        const int cnt = (int)(n - t * kLossTile);
+        for (int i = 0; i < cnt; ++i) s = s + row[i];
+      }
+    } else if (t + 1 < ntile) {
+      stage(t + 1, tiles + ((t + 1) & 1) * 64 * kLossStride);
+      if (t + 2 < ntile) fetch(t + 2);
+    }
+    __syncthreads();  // tile t consumed, tile t + 1 staged
+  }
+  return s;
+}
+
+__host__ __device__ inline bool lane_loss(int code) {
+  return code == kRmse || code == kRmseNorm || code == kStd || code == kRmseStd;
+}
+
+// loss_functions.c:41-50, 52-62, 129-145 and :33-34, one candidate per lane:
+// lane r's predictions are row r of P0.  Called by the whole block; the value
+// is that of the adding wave (wave kLossLoaders).
+__device__ float lane_loss_value(int code, const float* __restrict__ y,
+                                 const float* __restrict__ P0, int64_t n,
+                                 float* __restrict__ tile) {
+  const float* __restrict__ p = P0 + (int64_t)(threadIdx.x & 63) * n;
+  auto rmse = [&]() {
+    const float s = lane_seq_sum<false>(P0, n, tile, [&](int64_t q, float pq) {
+      const float t = y[q] - pq;
+      return t * t;
+    });
+    return (float)sqrt((double)(s / (float)n));
+  };
+  auto stdv = [&]() {
+    float mean = lane_seq_sum<false>(P0, n, tile, [&](int64_t q, float pq) { return y[q] - pq; });
+    mean = mean / (float)n;
+    const float t = (y[n - 1] - p[n - 1]) - mean;
+    const float s = (t * t) / (float)(n - 1);
+    return (float)sqrt((double)s);
+  };
+  switch (code) {
+    case kRmse: return rmse();
+    case kRmseNorm: {
+      const float s = lane_seq_sum<true>(P0, n, tile, [&](int64_t q, float pq) {
+        const float tv = y[q];
+        const float t = (float)((double)(tv - pq) / (1.0 + (double)tv));
+        return t * t;
+      });
+      return (float)sqrt((double)(s / (float)n));
+    }
+    case kStd: return stdv();
+    case kRmseStd: return rmse() + stdv();
+    default: return __int_as_float(0x7fc00000);
+  }
 }
 
 // loss_functions.c:129-145, keeping the assignment at :141 (only the last
@@ -427,8 +544,33 @@ __global__ __launch_bounds__(64) void k_loss(const float* __restrict__ P,
   const float* p = P + slot * n;
   float* scr = scratch ? scratch + slot * 2 * n : nullptr;  // median losses only
   for (int l = 0; l < nl; ++l) {
-    const float e = loss_value(losses[l], y, p, n, hist, scr);
+    const int code = losses[l];
+    if (lane_loss(code)) continue;  // k_loss_lanes
+    const float e = loss_value(code, y, p, n, hist, scr);
     if (threadIdx.x == 0) err[((int64_t)l * nk + m) * cpad + c] = e;
+  }
+}
+
+// The sequential-sum losses (rmse, rmse_norm, std, rmse_std): one candidate
+// per lane of the adding wave, 64 candidates per block, blockIdx.y = k value.
+__global__ __launch_bounds__(kLossThreads) void k_loss_lanes(const float* __restrict__ P,
+                                                   const float* __restrict__ y,
+                                                   const int* __restrict__ losses,
+                                                   const SelState* __restrict__ st,
+                                                   float* __restrict__ err, int64_t n, int cpad,
+                                                   int nk, int nl) {
+  __shared__ __align__(16) float tile[kLossLdsFloats];
+  if (st->stopped) return;
+  const int ncand = st->ncand;
+  if ((int)blockIdx.x * 64 >= ncand) return;
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63), m = blockIdx.y;
+  const bool adder = (threadIdx.x >> 6) == kLossLoaders;
+  const float* P0 = P + ((int64_t)m * cpad + blockIdx.x * 64) * n;  // 64 rows, all < cpad
+  for (int l = 0; l < nl; ++l) {
+    const int code = losses[l];
+    if (!lane_loss(code)) continue;
+    const float e = lane_loss_value(code, y, P0, n, tile);
+    if (adder && c < ncand) err[((int64_t)l * nk + m) * cpad + c] = e;
   }
 }
 
@@ -703,11 +845,24 @@ hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* 
   return hipGetLastError();
 }
 
+int loss_paths(const int* codes, int count) {
+  int paths = 0;
+  for (int l = 0; l < count; ++l) paths |= lane_loss(codes[l]) ? 2 : 1;
+  return paths;
+}
+
 hipError_t launch_loss(const float* P, const float* y, const int* losses, const SelState* st,
                        float* err, int64_t n, int cpad, int nk, int nl, int max_cand,
-                       float* scratch, hipStream_t s) {
-  dim3 g((unsigned)max_cand, (unsigned)nk);
-  hipLaunchKernelGGL(k_loss, g, dim3(64), 0, s, P, y, losses, st, err, n, cpad, nk, nl, scratch);
+                       float* scratch, int paths, hipStream_t s) {
+  if (paths & 1) {
+    dim3 g((unsigned)max_cand, (unsigned)nk);
+    hipLaunchKernelGGL(k_loss, g, dim3(64), 0, s, P, y, losses, st, err, n, cpad, nk, nl,
+                       scratch);
+  }
+  if (paths & 2) {
+    dim3 g((unsigned)((max_cand + 63) / 64), (unsigned)nk);
+    hipLaunchKernelGGL(k_loss_lanes, g, dim3(kLossThreads), 0, s, P, y, losses, st, err, n, cpad, nk, nl);
+  }
   return hipGetLastError();
 }
 

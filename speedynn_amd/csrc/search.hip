@@ -705,6 +705,172 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
   }
 }
 
+// Second forward step (one feature g chosen, so D_qj = fl((x_qg - x_jg)^2)
+// exactly, the first step's update of D = 0): walking outward from the
+// query's position in g's sorted column visits the points in ascending D
+// (non-decreasing on each side, the two sides merged), so the wave's bound
+// tightens after few points and the walk stops at the first point with
+// D > T = the largest K-th distance of the wave's live lanes (d >= D).  No D
+// row is read.  Insertion is by (distance, index), so the walk order does not
+// change the lists.  Windows of 64 sorted positions per side sit in VGPRs and
+// the next point is taken with readlane.  Needs finite X (st->nonfinite == 0;
+// otherwise the wave runs pruned_wave).  The wave mapping is k_search_pruned's.
+template <int K>
+__global__ __launch_bounds__(256) void k_search_walk(const SearchProblem a,
+                                                     const int* __restrict__ rank) {
+  __shared__ float lds[4 * K * 64];
+  const int wave = uniform(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int nblk = a.cpad / 64;
+  const int64_t per_xcd = (a.n + 7) / 8;
+  const int64_t local = (int64_t)(blockIdx.x / 8) * 4 + wave;
+  const int64_t qi = local % per_xcd;
+  const int cb = (int)(local / per_xcd);
+  const int64_t q = (int64_t)(blockIdx.x % 8) * per_xcd + qi;
+  if (q >= a.n || cb >= nblk) return;  // whole wave; no barriers
+  float* lab = lds + wave * (K * 64) + lane;
+  const SelState* st = a.st;
+  if (st->stopped) return;
+  if (st->nonfinite) {
+    pruned_wave<K, 1>(a, lab, q, cb);
+    return;
+  }
+  const int ncand = st->ncand;
+  if (cb * 64 >= ncand) return;
+  const int n = (int)a.n;
+  const int c = cb * 64 + lane;
+  const bool live = c < ncand;
+  const int g = st->fstar;  // the feature the first step chose
+  const float* __restrict__ key = a.skey + (int64_t)g * n;
+  const int* __restrict__ idx = a.sidx + (int64_t)g * n;
+  const int pos = rank[q];
+  const float xg = key[pos];
+  const float* __restrict__ Xl = a.Xc + cb * 64 + lane;
+  const uint32_t ucpad = (uint32_t)a.cpad;
+  const float xq = Xl[(uint32_t)q * ucpad];
+  const int npf = (int)a.npf;
+  const int flo = ((int)q / npf) * npf;
+  const int fspan = flo + npf < n ? npf : n - flo;
+  const float kInf = __int_as_float(0x7f800000);
+  constexpr float kScreen = 1.00000095367431640625f;  // 1 + 2^-20
+  constexpr float kTiny = 1.0e-36f;
+
+  float Ld[K];
+  int Li[K];
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    Ld[m] = kMaxF;
+    Li[m] = 0;
+  }
+  float lim = live ? kInf : -kInf;
+  float T = kInf;
+  // side windows: left holds positions [lw, lw + 64), right [rw, rw + 64)
+  int l = pos - 1, r = pos + 1;
+  int lw = l - 63, rw = r;
+  float kl = lw + lane >= 0 && lw + lane < n ? key[lw + lane] : 0.0f;
+  int il = lw + lane >= 0 && lw + lane < n ? idx[lw + lane] : 0;
+  float kr = rw + lane < n ? key[rw + lane] : 0.0f;
+  int ir = rw + lane < n ? idx[rw + lane] : 0;
+  auto dl_of = [&]() {  // D of the next left point (+inf past the end)
+    if (l < 0) return kInf;
+    if (l < lw) {
+      lw = l - 63;
+      kl = lw + lane >= 0 ? key[lw + lane] : 0.0f;
+      il = lw + lane >= 0 ? idx[lw + lane] : 0;
+    }
+    const float t = xg - lane_value(kl, l - lw);
+    return t * t;
+  };
+  auto dr_of = [&]() {
+    if (r >= n) return kInf;
+    if (r >= rw + 64) {
+      rw = r;
+      kr = rw + lane < n ? key[rw + lane] : 0.0f;
+      ir = rw + lane < n ? idx[rw + lane] : 0;
+    }
+    const float t = xg - lane_value(kr, r - rw);
+    return t * t;
+  };
+  float Dl = dl_of(), Dr = dr_of();
+  unsigned long long seen = 0;
+  bool done = false;
+  while (!done) {
+    constexpr int BATCH = 8;
+    int js[BATCH];
+    float dqs[BATCH], ts[BATCH];
+    bool pass[BATCH];
+    int cnt = 0;
+#pragma unroll
+    for (int u = 0; u < BATCH; ++u) {
+      js[u] = -1;
+      dqs[u] = 0.0f;
+      while (!done && js[u] < 0) {
+        if (l < 0 && r >= n) {  // both sides exhausted
+          done = true;
+          break;
+        }
+        const bool left = r >= n || (l >= 0 && Dl <= Dr);
+        const float Dn = left ? Dl : Dr;
+        if (!(Dn <= T)) {  // every later point has D > T: no list can change
+          done = true;
+          break;
+        }
+        const int j = left ? __builtin_amdgcn_readlane(il, l - lw)
+                           : __builtin_amdgcn_readlane(ir, r - rw);
+        if (left) {
+          --l;
+          Dl = dl_of();
+        } else {
+          ++r;
+          Dr = dr_of();
+        }
+        if ((uint32_t)(j - flo) < (uint32_t)fspan) continue;  // the query's fold
+        js[u] = j;
+        dqs[u] = Dn;
+        cnt = u + 1;
+      }
+      ts[u] = js[u] >= 0 ? Xl[(uint32_t)js[u] * ucpad] : 0.0f;
+    }
+    seen += cnt;
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < BATCH; ++u) {
+      ts[u] = xq - ts[u];
+      pass[u] = u < cnt && fmaf(ts[u], ts[u], dqs[u]) < lim;
+      any |= pass[u];
+    }
+    if (__ballot(any)) {
+#pragma unroll
+      for (int u = 0; u < BATCH; ++u) {
+        if (pass[u]) {
+          const float dd = dqs[u] + ts[u] * ts[u];
+          if (pair_less(dd, js[u], Ld[K - 1], Li[K - 1])) {
+            topk_insert_pair<K>(Ld, Li, dd, js[u]);
+            lim = Ld[K - 1] < kMaxF ? fmaf(Ld[K - 1], kScreen, kTiny) : kInf;
+          }
+        }
+      }
+    }
+    T = wave_max(live ? Ld[K - 1] : -kInf);
+  }
+  if (a.visited && lane == 0) {
+    const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
+    atomicAdd(a.visited, seen * (unsigned long long)nlive);
+  }
+  finish_query<K>(a, Li, lab, q, c, live);
+}
+
+// rank[sidx[g][p]] = p for the feature g the first step chose.
+__global__ __launch_bounds__(256) void k_rank(const int* __restrict__ sidx,
+                                              const SelState* __restrict__ st,
+                                              int* __restrict__ rank, int64_t n) {
+  if (st->stopped || st->nonfinite) return;
+  const int* col = sidx + (int64_t)st->fstar * n;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * blockDim.x)
+    rank[col[p]] = (int)p;
+}
+
 // ------------------------------------------------------ persistent batch kernels
 //
 // Many independent problems per launch: a global counter hands out items
@@ -975,6 +1141,32 @@ hipError_t launch_search_pruned(const SearchProblem& a, int variant, hipStream_t
 bool pruned_shared_applies(const SearchProblem& a) {
   const int K = search_list_size(a.kmax), nblk = a.cpad / 64;
   return K > 0 && K <= 50 && (nblk <= 8 || (nblk <= 16 && K <= 20)) && pruned_variant() == 0;
+}
+
+template <int K>
+static hipError_t launch_walk_k(const SearchProblem& a, const int* rank, hipStream_t s) {
+  const int64_t per_xcd = (a.n + 7) / 8;
+  const int64_t slice_blocks = (per_xcd * (a.cpad / 64) + 3) / 4;
+  hipLaunchKernelGGL(k_search_walk<K>, dim3((unsigned)(8 * slice_blocks)), dim3(256), 0, s, a,
+                     rank);
+  return hipGetLastError();
+}
+
+hipError_t launch_search_walk(const SearchProblem& a, const int* rank, hipStream_t s) {
+  switch (search_list_size(a.kmax)) {
+#define SNN_CASE(K, QW, W) \
+  case K: return launch_walk_k<K>(a, rank, s);
+    SNN_FOR_EACH_LIST_SIZE(SNN_CASE)
+#undef SNN_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_rank(const int* sidx, const SelState* st, int* rank, int64_t n, hipStream_t s) {
+  const int64_t g = (n + 255) / 256;
+  hipLaunchKernelGGL(k_rank, dim3((unsigned)(g < 4096 ? g : 4096)), dim3(256), 0, s, sidx, st,
+                     rank, n);
+  return hipGetLastError();
 }
 
 bool pruned_split_applies(const SearchProblem& a) {
