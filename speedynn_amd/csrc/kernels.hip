@@ -156,6 +156,8 @@ __global__ __launch_bounds__(256) void k_gram_exact(const float* __restrict__ XT
 // when f >= 0, else the device state's pending choice.  Streaming: float4
 // along each row, x_if read once per row.  COMP keeps the TwoSum/TwoProduct
 // error in Dlo; D itself stays bit-identical to the plain update.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 template <bool COMP, bool VEC>
 __global__ __launch_bounds__(256) void k_update(float* __restrict__ D, float* __restrict__ Dlo,
                                                 const float* __restrict__ XT, int64_t n,
@@ -170,25 +172,44 @@ __global__ __launch_bounds__(256) void k_update(float* __restrict__ D, float* __
     float* __restrict__ row = D + i * n;
     float* __restrict__ rlo = COMP ? Dlo + i * n : nullptr;
     if (VEC) {
-      for (int64_t j = (int64_t)threadIdx.x * 4; j < n; j += 256 * 4) {
-        const float4 xj = *reinterpret_cast<const float4*>(xf + j);
-        float4 dv = *reinterpret_cast<float4*>(row + j);
-        float4 lv;
-        if (COMP) lv = *reinterpret_cast<float4*>(rlo + j);
-        float* dp = &dv.x;
-        float* lp = &lv.x;
-        const float* xp = &xj.x;
+      // four float4 of the row in flight per thread; D streams through once
+      // (non-temporal), leaving the caches to the gathered feature values
+      auto upd = [&](v4f& dv, v4f& lv, const v4f xj) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const float t = xi - xp[e];
+          const float t = xi - xj[e];
           const float p = t * t;
           const float ps = mult > 0 ? p : -p;
-          const float s = dp[e] + ps;
-          if (COMP) lp[e] += comp_error(xi, xp[e], t, p, dp[e], ps, s, mult < 0);
-          dp[e] = s;
+          const float s = dv[e] + ps;
+          if (COMP) lv[e] += comp_error(xi, xj[e], t, p, dv[e], ps, s, mult < 0);
+          dv[e] = s;
         }
-        *reinterpret_cast<float4*>(row + j) = dv;
-        if (COMP) *reinterpret_cast<float4*>(rlo + j) = lv;
+      };
+      constexpr int U = 4;
+      int64_t j = (int64_t)threadIdx.x * 4;
+      for (; j + (U - 1) * 1024 < n; j += U * 1024) {
+        v4f dv[U], lv[U], xj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          xj[u] = *reinterpret_cast<const v4f*>(xf + j + u * 1024);
+          dv[u] = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j + u * 1024));
+          if (COMP) lv[u] = *reinterpret_cast<v4f*>(rlo + j + u * 1024);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          upd(dv[u], lv[u], xj[u]);
+          __builtin_nontemporal_store(dv[u], reinterpret_cast<v4f*>(row + j + u * 1024));
+          if (COMP) *reinterpret_cast<v4f*>(rlo + j + u * 1024) = lv[u];
+        }
+      }
+      for (; j < n; j += 1024) {
+        v4f lv = {0.f, 0.f, 0.f, 0.f};
+        const v4f xj = *reinterpret_cast<const v4f*>(xf + j);
+        v4f dv = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j));
+        if (COMP) lv = *reinterpret_cast<v4f*>(rlo + j);
+        upd(dv, lv, xj);
+        __builtin_nontemporal_store(dv, reinterpret_cast<v4f*>(row + j));
+        if (COMP) *reinterpret_cast<v4f*>(rlo + j) = lv;
       }
     } else {
       for (int64_t j = threadIdx.x; j < n; j += 256) {

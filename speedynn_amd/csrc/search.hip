@@ -213,6 +213,25 @@ __device__ __forceinline__ void kth_insert(float (&B)[K], float v) {
   B[0] = v < B[0] ? v : B[0];
 }
 
+// Smallest float above a finite v.
+__device__ __forceinline__ float next_up(float v) {
+  const unsigned int b = __float_as_uint(v);
+  if (b == 0x80000000u) return __uint_as_float(1u);  // -0: the smallest subnormal
+  return __uint_as_float((int)b >= 0 ? b + 1u : b - 1u);
+}
+
+// Initial screen limit of a lane with bound tau (the K-th smallest exact
+// distance over K eligible points, so no point with d > tau is in its final
+// list and such points need not be offered; +inf: no bound, -inf: padding
+// lane).  Forward screens compare fma(t, t, D) with tau widened by 2^-20 (plus
+// 1e-36), which passes every d <= tau; backward tests are exact, d <= tau.
+template <int MULT>
+__device__ __forceinline__ float screen_limit(float tau) {
+  const float kInf = __int_as_float(0x7f800000);
+  if (tau == kInf || tau == -kInf) return tau;
+  return MULT > 0 ? fmaf(tau, 1.00000095367431640625f, 1.0e-36f) : next_up(tau);
+}
+
 // Bound of one lane (candidate): the K-th smallest exact distance over the
 // query's seeds, which k_seed_scan lists by ascending D.  Forward distances
 // are >= D, so once the next seed's D exceeds every live lane's current K-th
@@ -314,14 +333,17 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* lab, 
     Ld[m] = kMaxF;
     Li[m] = 0;
   }
-  // screen limit: live lanes pass every point until their list is full
-  // (+inf); padding lanes never pass (-inf)
-  float lim = live ? kInf : -kInf;
+  // screen limit: d <= thr until the list is full, then its K-th distance;
+  // padding lanes never pass (-inf)
+  float lim = screen_limit<MULT>(thr);
   unsigned long long seen = 0;
 
-  // D (and M) of 64 points per chunk; a window of 4 chunks is in flight
-  auto ldD = [&](int64_t j) { return j < n ? Dq[j] : kInf; };
-  auto ldM = [&](int64_t j) { return (MULT < 0 && j < n) ? Mq[j] : 0.0f; };
+  // D (and M) of 64 points per chunk; a window of 4 chunks is in flight.  The
+  // rows stream through once (non-temporal), leaving the caches to Xc.
+  auto ldD = [&](int64_t j) { return j < n ? __builtin_nontemporal_load(Dq + j) : kInf; };
+  auto ldM = [&](int64_t j) {
+    return (MULT < 0 && j < n) ? __builtin_nontemporal_load(Mq + j) : 0.0f;
+  };
   float d0 = ldD(lane), d1 = ldD(64 + lane), d2 = ldD(128 + lane), d3 = ldD(192 + lane);
   float m0 = ldM(lane), m1 = ldM(64 + lane), m2 = ldM(128 + lane), m3 = ldM(192 + lane);
   const uint32_t ucpad = (uint32_t)cpad;  // n * cpad < 2^31 (checked on the host)
@@ -378,8 +400,9 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* lab, 
             const float dd = dist(dqs[u], p);
             if (dd < Ld[K - 1]) {
               topk_insert<K>(Ld, Li, dd, js[u]);
-              lim = MULT > 0 ? (Ld[K - 1] < kMaxF ? fmaf(Ld[K - 1], kScreen, kTiny) : kInf)
-                             : Ld[K - 1];
+              lim = fminf(lim, MULT > 0 ? (Ld[K - 1] < kMaxF ? fmaf(Ld[K - 1], kScreen, kTiny)
+                                                              : kInf)
+                                        : Ld[K - 1]);
               changed = true;
             }
           }
@@ -548,7 +571,10 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
   const int q = blockIdx.x;
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int cb = wave + W * blockIdx.y;  // several workgroups per query split the blocks
+  // several workgroups per query split its candidate blocks: blockIdx.y = part,
+  // so all queries' first parts run before the second parts and the gathers
+  // of each half of the run stay within half of Xc's columns
+  const int cb = wave + W * blockIdx.y;
   const bool active = cb * 64 < ncand && cb < a.cpad / 64;
   const int c = cb * 64 + lane;
   const bool live = active && c < ncand;
@@ -576,11 +602,11 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
     Ld[m] = kMaxF;
     Li[m] = 0;
   }
-  float lim = live ? kInf : -kInf;
+  float lim = screen_limit<MULT>(thr);  // as in pruned_wave
   unsigned long long seen = 0;
 
-  auto ldD = [&](int j) { return j < n ? Dq[j] : kInf; };
-  auto ldM = [&](int j) { return (MULT < 0 && j < n) ? Mq[j] : 0.0f; };
+  auto ldD = [&](int j) { return j < n ? __builtin_nontemporal_load(Dq + j) : kInf; };
+  auto ldM = [&](int j) { return (MULT < 0 && j < n) ? __builtin_nontemporal_load(Mq + j) : 0.0f; };
   const int t0 = threadIdx.x;
   float dn[R], mn[R];  // the next chunk's D (and M), loaded one chunk ahead
 #pragma unroll
@@ -678,8 +704,9 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
               const float dd = dist(dqs[u], ts[u] * ts[u]);
               if (dd < Ld[K - 1]) {
                 topk_insert<K>(Ld, Li, dd, js[u]);
-                lim = MULT > 0 ? (Ld[K - 1] < kMaxF ? fmaf(Ld[K - 1], kScreen, kTiny) : kInf)
-                               : Ld[K - 1];
+                lim = fminf(lim, MULT > 0 ? (Ld[K - 1] < kMaxF ? fmaf(Ld[K - 1], kScreen, kTiny)
+                                                                : kInf)
+                                          : Ld[K - 1]);
                 changed = true;
               }
             }
@@ -983,7 +1010,7 @@ __global__ __launch_bounds__(256) void k_seed_scan(const float* __restrict__ D,
   for (; j + 7 * 64 < n; j += 8 * 64) {  // eight loads in flight per lane
     float v[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = Dq[j + u * 64];
+    for (int u = 0; u < 8; ++u) v[u] = __builtin_nontemporal_load(Dq + j + u * 64);
 #pragma unroll
     for (int u = 0; u < 8; ++u) offer(v[u], j + u * 64);
   }
@@ -1078,7 +1105,8 @@ static hipError_t launch_pruned_k(const SearchProblem& a, int variant, hipStream
   }
   // variant 3: the shared scan split over two workgroups per query (half the
   // candidate blocks each; the row is read twice, but two workgroups share a
-  // CU, so one's barrier waits hide behind the other's work)
+  // CU, so one's barrier waits hide behind the other's work, and each half of
+  // the run gathers from half of Xc's columns)
   if (v == 3 && K <= 50 && nblk > 4 && nblk <= 16) {
     if constexpr (K <= 50) {
       if (nblk <= 8) launch_pruned_shared<K, 4>(a, s);
