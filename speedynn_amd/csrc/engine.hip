@@ -154,6 +154,20 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int
   if (rank) cands[nc++] = 4;
   int variant = 0;
   bool tune = false;
+  // SPEEDYNN_PRUNED=wave|shared|split|walk runs that kernel wherever it
+  // applies (read per call, for measurements and the per-kernel tests)
+  const char* fe = std::getenv("SPEEDYNN_PRUNED");
+  if (fe && *fe) {
+    int f = !std::strcmp(fe, "wave") ? 1 : !std::strcmp(fe, "shared") ? 2
+          : !std::strcmp(fe, "split") ? 3 : !std::strcmp(fe, "walk") ? 4 : 0;
+    for (int i = 0; i < nc; ++i)
+      if (cands[i] == f) variant = f;
+    if (variant == 0 && f != 4) variant = 1;  // the per-wave kernel applies everywhere
+    HIPCHK(launch_variant(sl, variant, rank, s));
+    const int v = variant ? variant : (sl.mult > 0 || !snn::pruned_shared_applies(sl) ? 1 : 2);
+    *used |= 1 << (v - 1);
+    return SNN_OK;
+  }
   if (!off && nc > 1) {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tuned.find(key);
@@ -326,12 +340,19 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   constexpr int kMarks = 5;
   int used_variant = 0;
   std::vector<hipEvent_t> marks;
+  // backward: M is recomputed over the remaining candidates each time their
+  // number has halved (d/2, d/4, ...), keeping the bound fl(D - M) tight
+  int m_refresh = d / 2;
   for (int step = 0; step < pb.max_steps; ++step) {
     hipEvent_t a = tm.make(), b = tm.make(), b2 = tm.make(), c = tm.make(), u = tm.make();
     HIPCHK(hipEventRecord(a, s));
     HIPCHK(snn::launch_compact(dX, ctx->cand.as<int>(), st, ctx->Xc.as<float>(), n, d, pb.cpad, s));
     HIPCHK(hipEventRecord(b, s));
     const bool pruned_step = pruned && (step > 0 || !fwd);
+    if (pruned && !fwd && step > 0 && d - step <= m_refresh) {
+      HIPCHK(snn::launch_max_square(ctx->Xc.as<float>(), st, ctx->M.as<float>(), n, pb.cpad, s));
+      m_refresh = (d - step) / 2;
+    }
     if (pruned_step) HIPCHK(snn::launch_seed_scan(D, st, ctx->seeds.as<int>(), n, pb.npf, s));
     HIPCHK(hipEventRecord(b2, s));
     if (pruned_step) {

@@ -9,6 +9,10 @@ hipError_t launch_fill_u64(unsigned long long* p, int64_t count, hipStream_t s);
 hipError_t launch_transpose(const float* X, float* XT, int64_t n, int d, hipStream_t s);
 hipError_t launch_gram_exact(const float* XT, float* D, float* Dlo, float* M, int64_t n, int d,
                              hipStream_t s);
+// Backward pruned search: M[i][j] = max over the state's remaining candidates
+// (Xc columns c < ncand) of fl((Xc[i][c] - Xc[j][c])^2).
+hipError_t launch_max_square(const float* Xc, const SelState* st, float* M, int64_t n, int cpad,
+                             hipStream_t s);
 hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const SelState* st,
                          int f, int mult, hipStream_t s);
 hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* Xc, int64_t n,

@@ -147,6 +147,70 @@ __global__ __launch_bounds__(256) void k_gram_exact(const float* __restrict__ XT
   }
 }
 
+// M refreshed over the remaining candidates of a backward selection:
+// M[i][j] = max over c < ncand of fl((Xc[i][c] - Xc[j][c])^2), the same
+// squares the search subtracts, so fl(D - M) stays a lower bound of every
+// candidate's distance and tightens as candidates go (the M of the full
+// matrix keeps the largest term of features already removed).  Tiles as in
+// k_gram_exact, from the point-major candidate columns.
+__global__ __launch_bounds__(256) void k_max_square(const float* __restrict__ Xc,
+                                                    const SelState* __restrict__ st,
+                                                    float* __restrict__ M, int64_t n, int cpad) {
+  constexpr int KC = 16;
+  __shared__ float As[KC][64];
+  __shared__ float Bs[KC][64];
+  if (st->stopped) return;
+  const int ncand = st->ncand;
+  const int64_t i0 = (int64_t)blockIdx.y * 64;
+  const int64_t j0 = (int64_t)blockIdx.x * 64;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  float mx[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) mx[a][b] = 0.0f;
+  for (int c0 = 0; c0 < ncand; c0 += KC) {
+    for (int t = threadIdx.x; t < KC * 64; t += 256) {
+      const int r = t >> 4, kk = t & 15;  // 16 consecutive columns of a row
+      const int c = c0 + kk;
+      float av = 0.0f, bv = 0.0f;
+      if (c < cpad) {
+        if (i0 + r < n) av = Xc[(i0 + r) * cpad + c];
+        if (j0 + r < n) bv = Xc[(j0 + r) * cpad + c];
+      }
+      As[kk][r] = av;
+      Bs[kk][r] = bv;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < KC; ++kk) {
+      float a4[4], b4[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) a4[a] = As[kk][ty * 4 + a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) b4[b] = Bs[kk][tx * 4 + b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const float t = a4[a] - b4[b];
+          mx[a][b] = fmaxf(mx[a][b], t * t);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int64_t i = i0 + ty * 4 + a;
+    if (i >= n) continue;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int64_t j = j0 + tx * 4 + b;
+      if (j < n) M[i * n + j] = mx[a][b];
+    }
+  }
+}
+
 // --------------------------------------------------------- (b) update
 //
 // D[i][j] = fl(D[i][j] + mult * fl((x_if - x_jf)^2)) over the full matrix
@@ -842,6 +906,13 @@ hipError_t launch_gram_exact(const float* XT, float* D, float* Dlo, float* M, in
   else if (Dlo) hipLaunchKernelGGL((k_gram_exact<true, false>), g, b, 0, s, XT, D, Dlo, M, n, d);
   else if (M) hipLaunchKernelGGL((k_gram_exact<false, true>), g, b, 0, s, XT, D, Dlo, M, n, d);
   else hipLaunchKernelGGL((k_gram_exact<false, false>), g, b, 0, s, XT, D, Dlo, M, n, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_square(const float* Xc, const SelState* st, float* M, int64_t n, int cpad,
+                             hipStream_t s) {
+  dim3 g((unsigned)((n + 63) / 64), (unsigned)((n + 63) / 64));
+  hipLaunchKernelGGL(k_max_square, g, dim3(256), 0, s, Xc, st, M, n, cpad);
   return hipGetLastError();
 }
 

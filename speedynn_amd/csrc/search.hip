@@ -575,6 +575,7 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
   // so all queries' first parts run before the second parts and the gathers
   // of each half of the run stay within half of Xc's columns
   const int cb = wave + W * blockIdx.y;
+  if ((int)blockIdx.y * W * 64 >= ncand) return;  // no live block in this part: whole workgroup
   const bool active = cb * 64 < ncand && cb < a.cpad / 64;
   const int c = cb * 64 + lane;
   const bool live = active && c < ncand;
@@ -1050,20 +1051,6 @@ static hipError_t launch_dense_k(const SearchProblem& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Pruned-search variant: 0 = pick by shape, 1 = a wave per (query, candidate
-// block), 2 = a workgroup per query sharing the row scan.  Overridable with
-// SPEEDYNN_PRUNED=wave|shared for measurements.
-static int pruned_variant() {
-  static const int v = [] {
-    const char* e = std::getenv("SPEEDYNN_PRUNED");
-    if (!e) return 0;
-    if (!std::strcmp(e, "wave")) return 1;
-    if (!std::strcmp(e, "shared")) return 2;
-    return 0;
-  }();
-  return v;
-}
-
 int backward_seed_count(int kmax) {
   static const int forced = [] {
     const char* e = std::getenv("SPEEDYNN_NSEED");
@@ -1092,8 +1079,7 @@ static hipError_t launch_pruned_k(const SearchProblem& a, int variant, hipStream
   // bounds differ widely when feature scales do, and a shared workgroup waits
   // for its loosest block), backward the shared scan (two rows per query,
   // similar bounds)
-  const int forced = pruned_variant();
-  const int v = forced ? forced : variant;
+  const int v = variant;
   const bool shared = K <= 50 && W > 0 && (v == 2 || (v == 0 && a.mult < 0));
   if (shared) {
     if constexpr (K <= 50) {
@@ -1168,7 +1154,7 @@ hipError_t launch_search_pruned(const SearchProblem& a, int variant, hipStream_t
 
 bool pruned_shared_applies(const SearchProblem& a) {
   const int K = search_list_size(a.kmax), nblk = a.cpad / 64;
-  return K > 0 && K <= 50 && (nblk <= 8 || (nblk <= 16 && K <= 20)) && pruned_variant() == 0;
+  return K > 0 && K <= 50 && (nblk <= 8 || (nblk <= 16 && K <= 20));
 }
 
 template <int K>
@@ -1199,7 +1185,7 @@ hipError_t launch_rank(const int* sidx, const SelState* st, int* rank, int64_t n
 
 bool pruned_split_applies(const SearchProblem& a) {
   const int K = search_list_size(a.kmax), nblk = a.cpad / 64;
-  return K > 0 && K <= 50 && nblk > 4 && nblk <= 16 && pruned_variant() == 0;
+  return K > 0 && K <= 50 && nblk > 4 && nblk <= 16;
 }
 
 // Dense configuration (queries per wave, waves per block) of a list size.

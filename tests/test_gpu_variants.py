@@ -1,0 +1,67 @@
+"""Every pruned-search kernel against the CPU oracle: SPEEDYNN_PRUNED forces
+one kernel (per-wave, shared scan, split shared scan, sorted-column walk) for
+every step where it applies, so each is checked on its own rather than only
+when the autotuner picks it.  Shapes are chosen so that each kernel applies:
+more than four candidate blocks for the split scan, a second forward step with
+finite X for the walk, and a long backward run in which the bound matrix M is
+recomputed over the remaining candidates (at d/2 and d/4)."""
+import numpy as np
+import pytest
+
+from conftest import assert_same_result
+from oracle import oracle as O
+from speedynn_amd import _lib, datasets as ds
+
+pytestmark = pytest.mark.gpu
+
+LOSS = {"rmse": 0, "zero_one": 4}
+
+CASES = [
+    # seed, n, d, k, method, model, num_features
+    (81, 400, 300, 10, "forward", "classification", 4),
+    (82, 200, 300, 5, "backward", "regression", 240),
+    (83, 300, 150, 5, "backward", "regression", 20),
+    (84, 350, 70, 20, "forward", "regression", 5),
+]
+
+
+@pytest.fixture(scope="module")
+def engine():
+    print("engine: creating context on device 0", flush=True)
+    return _lib.Engine(0)
+
+
+def _data(seed, n, d, model):
+    if model == "classification":
+        X, y, _ = ds.classification_argmax(seed, n, d, 4, 3, 0.1)
+    else:
+        X, y, _ = ds.regression_sum(seed, n, d, 4, 0.5)
+    return X, y
+
+
+@pytest.mark.parametrize("kernel", ["wave", "shared", "split", "walk"])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"seed{c[0]}")
+def test_forced_pruned_kernel_matches_oracle(engine, monkeypatch, case, kernel):
+    seed, n, d, k, method, model, nf = case
+    X, y = _data(seed, n, d, model)
+    loss = "zero_one" if model == "classification" else "rmse"
+    print(f"kernel={kernel} seed={seed} n={n} d={d} {method}", flush=True)
+    o = O.select(X, y, [k], nf, n, [loss], False, model, method, record_candidates=True)
+    monkeypatch.setenv("SPEEDYNN_PRUNED", kernel)
+    g = engine.select(X, y, [k], [LOSS[loss]], method=1 if method == "forward" else 0,
+                      model_type=1 if model == "classification" else 0, k_fold=n,
+                      num_features=nf, form=1, candidate_errors=True)
+    assert_same_result(g, o)
+    steps = g["steps"]
+    np.testing.assert_array_equal(g["candidate_errors"][:steps].view(np.uint32),
+                                  o["candidate_errors"][:steps].view(np.uint32))
+    used = g["pruned_kernel"]
+    want = {"wave": 1, "shared": 2, "split": 4, "walk": 8}[kernel]
+    applies = {
+        "wave": True,
+        "shared": True,
+        "split": d > 256,
+        "walk": method == "forward",
+    }[kernel]
+    if applies:
+        assert used & want, (kernel, used)
