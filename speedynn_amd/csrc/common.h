@@ -90,4 +90,52 @@ __device__ __forceinline__ float lane_value(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// Pruning seeds of one query, kept by one wave: each lane holds the two
+// smallest D it was offered (b0 <= b1, point indices i0, i1; -1 and +inf when
+// fewer).  offer() skips the query's fold [flo, fhi) and keeps the earlier
+// point among equal values.
+struct SeedPair {
+  float b0, b1;
+  int i0, i1;
+  __device__ __forceinline__ void reset() {
+    b0 = b1 = __int_as_float(0x7f800000);
+    i0 = i1 = -1;
+  }
+  __device__ __forceinline__ void offer(float v, int64_t j, int64_t flo, int64_t fhi) {
+    if (j >= flo && j < fhi) return;
+    if (v < b1) {
+      if (v < b0) {
+        b1 = b0;
+        i1 = i0;
+        b0 = v;
+        i0 = (int)j;
+      } else {
+        b1 = v;
+        i1 = (int)j;
+      }
+    }
+  }
+  // The wave's 128 entries written to row[0..128) by ascending D (ties by
+  // slot: lane k's pair is slots k and 64 + k), on a total order of the float
+  // bits, so the search can stop at the first seed whose D is too large.
+  __device__ __forceinline__ void write_ranked(int* row) const {
+    const int lane = threadIdx.x & 63;
+    auto sortable = [](float v) {
+      const unsigned int b = __float_as_uint(v);
+      return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    };
+    const unsigned int k0 = sortable(b0), k1 = sortable(b1);
+    int r0 = 0, r1 = 0;
+    for (int k = 0; k < 64; ++k) {
+      const unsigned int a0 = __shfl(k0, k), a1 = __shfl(k1, k);  // slots k and 64 + k
+      r0 += (a0 < k0 || (a0 == k0 && k < lane)) ? 1 : 0;
+      r0 += a1 < k0 ? 1 : 0;
+      r1 += a0 <= k1 ? 1 : 0;
+      r1 += (a1 < k1 || (a1 == k1 && k < lane)) ? 1 : 0;
+    }
+    row[r0] = i0;
+    row[r1] = i1;
+  }
+};
+
 }  // namespace snn

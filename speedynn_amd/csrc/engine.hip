@@ -343,6 +343,9 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   // backward: M is recomputed over the remaining candidates each time their
   // number has halved (d/2, d/4, ...), keeping the bound fl(D - M) tight
   int m_refresh = d / 2;
+  // precision form: each update also writes the next step's seeds (rows of a
+  // multiple of 4 floats); the first pruned step of a backward run scans them
+  const bool fused_seeds = pruned && n % 4 == 0;
   for (int step = 0; step < pb.max_steps; ++step) {
     hipEvent_t a = tm.make(), b = tm.make(), b2 = tm.make(), c = tm.make(), u = tm.make();
     HIPCHK(hipEventRecord(a, s));
@@ -353,7 +356,9 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
       HIPCHK(snn::launch_max_square(ctx->Xc.as<float>(), st, ctx->M.as<float>(), n, pb.cpad, s));
       m_refresh = (d - step) / 2;
     }
-    if (pruned_step) HIPCHK(snn::launch_seed_scan(D, st, ctx->seeds.as<int>(), n, pb.npf, s));
+    // seeds: the previous step's update wrote them where it was fused
+    if (pruned_step && !(fused_seeds && step > 0))
+      HIPCHK(snn::launch_seed_scan(D, st, ctx->seeds.as<int>(), n, pb.npf, s));
     HIPCHK(hipEventRecord(b2, s));
     if (pruned_step) {
       // the second forward step may walk the first chosen feature's sorted column
@@ -380,7 +385,11 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
                               ctx->errors.as<float>(), ctx->val.as<float>(), ctx->bestk.as<int>(),
                               cand_err, ctx->perm.as<int>(), s));
     HIPCHK(hipEventRecord(u, s));
-    HIPCHK(snn::launch_update(D, Dlo, ctx->XT.as<float>(), n, st, -1, fwd ? 1 : -1, s));
+    if (fused_seeds)
+      HIPCHK(snn::launch_update_seeds(D, Dlo, ctx->XT.as<float>(), n, st, fwd ? 1 : -1,
+                                      ctx->seeds.as<int>(), pb.npf, s));
+    else
+      HIPCHK(snn::launch_update(D, Dlo, ctx->XT.as<float>(), n, st, -1, fwd ? 1 : -1, s));
     for (hipEvent_t e : {a, b, b2, c, u}) marks.push_back(e);
   }
   hipEvent_t e2 = tm.make();

@@ -13,6 +13,11 @@ hipError_t launch_gram_exact(const float* XT, float* D, float* Dlo, float* M, in
 // (Xc columns c < ncand) of fl((Xc[i][c] - Xc[j][c])^2).
 hipError_t launch_max_square(const float* Xc, const SelState* st, float* M, int64_t n, int cpad,
                              hipStream_t s);
+// The update of the device state's pending feature fused with the seed scan
+// of the next step (seeds as launch_seed_scan writes them); n % 4 == 0.
+hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
+                               const SelState* st, int mult, int* seeds, int64_t npf,
+                               hipStream_t s);
 hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const SelState* st,
                          int f, int mult, hipStream_t s);
 hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* Xc, int64_t n,

@@ -290,6 +290,73 @@ __global__ __launch_bounds__(256) void k_update(float* __restrict__ D, float* __
   }
 }
 
+// The update fused with the next step's seed scan (n % 4 == 0, feature from
+// the device state): one wave per row i updates the row exactly as k_update
+// does and keeps, per lane, the two smallest new D_ij outside i's fold (lane
+// L is offered the points 4L .. 4L + 3 of every 256); the row's 128 seeds are
+// then written ranked by D, as k_seed_scan writes them.  Saves the seed
+// scan's full read of D on every pruned step.
+template <bool COMP>
+__global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
+                                                      float* __restrict__ Dlo,
+                                                      const float* __restrict__ XT, int64_t n,
+                                                      const SelState* __restrict__ st, int mult,
+                                                      int* __restrict__ seeds, int64_t npf) {
+  if (!st->pending) return;
+  const int f = st->fstar;
+  const int lane = threadIdx.x & 63;
+  const float* __restrict__ xf = XT + (int64_t)f * n;
+  for (int64_t i = (int64_t)blockIdx.x * 4 + uniform(threadIdx.x >> 6); i < n;
+       i += (int64_t)gridDim.x * 4) {
+    const float xi = xf[i];
+    float* __restrict__ row = D + i * n;
+    float* __restrict__ rlo = COMP ? Dlo + i * n : nullptr;
+    const int64_t flo = (i / npf) * npf;
+    const int64_t fhi = flo + npf < n ? flo + npf : n;
+    SeedPair sp;
+    sp.reset();
+    auto upd = [&](v4f& dv, v4f& lv, const v4f xj, int64_t j) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float t = xi - xj[e];
+        const float p = t * t;
+        const float ps = mult > 0 ? p : -p;
+        const float s = dv[e] + ps;
+        if (COMP) lv[e] += comp_error(xi, xj[e], t, p, dv[e], ps, s, mult < 0);
+        dv[e] = s;
+        sp.offer(s, j + e, flo, fhi);
+      }
+    };
+    constexpr int U = 4;
+    int64_t j = (int64_t)lane * 4;
+    for (; j + (U - 1) * 256 < n; j += U * 256) {
+      v4f dv[U], lv[U], xj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        xj[u] = *reinterpret_cast<const v4f*>(xf + j + u * 256);
+        dv[u] = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j + u * 256));
+        if (COMP) lv[u] = *reinterpret_cast<v4f*>(rlo + j + u * 256);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        upd(dv[u], lv[u], xj[u], j + u * 256);
+        __builtin_nontemporal_store(dv[u], reinterpret_cast<v4f*>(row + j + u * 256));
+        if (COMP) *reinterpret_cast<v4f*>(rlo + j + u * 256) = lv[u];
+      }
+    }
+    for (; j < n; j += 256) {
+      v4f lv = {0.f, 0.f, 0.f, 0.f};
+      const v4f xj = *reinterpret_cast<const v4f*>(xf + j);
+      v4f dv = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j));
+      if (COMP) lv = *reinterpret_cast<v4f*>(rlo + j);
+      upd(dv, lv, xj, j);
+      __builtin_nontemporal_store(dv, reinterpret_cast<v4f*>(row + j));
+      if (COMP) *reinterpret_cast<v4f*>(rlo + j) = lv;
+    }
+    sp.write_ranked(seeds + i * 128);
+  }
+}
+
 // ------------------------------------------------------- candidate gather
 
 // Xc[j][c] = X[j][cand[c]] for c < ncand, 0 for the padding slots.
@@ -927,6 +994,21 @@ hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const
     if (vec) hipLaunchKernelGGL((k_update<false, true>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st, f, mult);
     else hipLaunchKernelGGL((k_update<false, false>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st, f, mult);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
+                               const SelState* st, int mult, int* seeds, int64_t npf,
+                               hipStream_t s) {
+  if (n % 4) return hipErrorInvalidValue;
+  const int64_t rows = (n + 3) / 4;
+  const int g = (int)(rows < 8192 ? rows : 8192);
+  if (Dlo)
+    hipLaunchKernelGGL((k_update_seeds<true>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st, mult,
+                       seeds, npf);
+  else
+    hipLaunchKernelGGL((k_update_seeds<false>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st, mult,
+                       seeds, npf);
   return hipGetLastError();
 }
 

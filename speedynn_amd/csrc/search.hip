@@ -990,49 +990,18 @@ __global__ __launch_bounds__(256) void k_seed_scan(const float* __restrict__ D,
   const float* __restrict__ Dq = D + q * n;
   const int64_t flo = (q / npf) * npf;
   const int64_t fhi = flo + npf < n ? flo + npf : n;
-  const float kInf = __int_as_float(0x7f800000);
-  float b0 = kInf, b1 = kInf;
-  int i0 = -1, i1 = -1;
-  auto offer = [&](float v, int64_t j) {
-    if (j >= flo && j < fhi) return;
-    if (v < b1) {
-      if (v < b0) {
-        b1 = b0;
-        i1 = i0;
-        b0 = v;
-        i0 = (int)j;
-      } else {
-        b1 = v;
-        i1 = (int)j;
-      }
-    }
-  };
+  SeedPair sp;
+  sp.reset();
   int64_t j = lane;
   for (; j + 7 * 64 < n; j += 8 * 64) {  // eight loads in flight per lane
     float v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) v[u] = __builtin_nontemporal_load(Dq + j + u * 64);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) offer(v[u], j + u * 64);
+    for (int u = 0; u < 8; ++u) sp.offer(v[u], j + u * 64, flo, fhi);
   }
-  for (; j < n; j += 64) offer(Dq[j], j);
-  // list the 128 seeds by ascending D (ties by slot): rank of each of the
-  // lane's two entries among all, on a total order of the float bits
-  auto sortable = [](float v) {
-    const unsigned int b = __float_as_uint(v);
-    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-  };
-  const unsigned int k0 = sortable(b0), k1 = sortable(b1);
-  int r0 = 0, r1 = 0;
-  for (int k = 0; k < 64; ++k) {
-    const unsigned int a0 = __shfl(k0, k), a1 = __shfl(k1, k);  // slots k and 64 + k
-    r0 += (a0 < k0 || (a0 == k0 && k < lane)) ? 1 : 0;
-    r0 += a1 < k0 ? 1 : 0;
-    r1 += a0 <= k1 ? 1 : 0;
-    r1 += (a1 < k1 || (a1 == k1 && k < lane)) ? 1 : 0;
-  }
-  seeds[q * 128 + r0] = i0;
-  seeds[q * 128 + r1] = i1;
+  for (; j < n; j += 64) sp.offer(Dq[j], j, flo, fhi);
+  sp.write_ranked(seeds + q * 128);
 }
 
 // ============================================================ launchers
