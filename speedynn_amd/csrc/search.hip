@@ -792,94 +792,81 @@ __global__ __launch_bounds__(256) void k_search_walk(const SearchProblem a,
   }
   float lim = live ? kInf : -kInf;
   float T = kInf;
-  // side windows: left holds positions [lw, lw + 64), right [rw, rw + 64)
-  int l = pos - 1, r = pos + 1;
-  int lw = l - 63, rw = r;
-  float kl = lw + lane >= 0 && lw + lane < n ? key[lw + lane] : 0.0f;
-  int il = lw + lane >= 0 && lw + lane < n ? idx[lw + lane] : 0;
-  float kr = rw + lane < n ? key[rw + lane] : 0.0f;
-  int ir = rw + lane < n ? idx[rw + lane] : 0;
-  auto dl_of = [&]() {  // D of the next left point (+inf past the end)
-    if (l < 0) return kInf;
-    if (l < lw) {
-      lw = l - 63;
-      kl = lw + lane >= 0 ? key[lw + lane] : 0.0f;
-      il = lw + lane >= 0 ? idx[lw + lane] : 0;
-    }
-    const float t = xg - lane_value(kl, l - lw);
-    return t * t;
-  };
-  auto dr_of = [&]() {
-    if (r >= n) return kInf;
-    if (r >= rw + 64) {
-      rw = r;
-      kr = rw + lane < n ? key[rw + lane] : 0.0f;
-      ir = rw + lane < n ? idx[rw + lane] : 0;
-    }
-    const float t = xg - lane_value(kr, r - rw);
-    return t * t;
-  };
-  float Dl = dl_of(), Dr = dr_of();
   unsigned long long seen = 0;
-  bool done = false;
-  while (!done) {
-    constexpr int BATCH = 8;
-    int js[BATCH];
-    float dqs[BATCH], ts[BATCH];
-    bool pass[BATCH];
-    int cnt = 0;
+  // Windows of 64 sorted positions, one lane per position: left windows end
+  // at lo - 1 (lane 63 nearest the query), right windows start at hi (lane 0
+  // nearest).  A window's D values come from the sorted keys; its points
+  // within T outside the query's fold are visited as in the row scan (a
+  // ballot, then eight gathers in flight).  A side stops at the first window
+  // whose nearest point has D > T, since D does not decrease outward.
+  int lo = pos, hi = pos + 1;
+  bool lact = lo > 0, ract = hi < n;
+  while (lact || ract) {
 #pragma unroll
-    for (int u = 0; u < BATCH; ++u) {
-      js[u] = -1;
-      dqs[u] = 0.0f;
-      while (!done && js[u] < 0) {
-        if (l < 0 && r >= n) {  // both sides exhausted
-          done = true;
-          break;
-        }
-        const bool left = r >= n || (l >= 0 && Dl <= Dr);
-        const float Dn = left ? Dl : Dr;
-        if (!(Dn <= T)) {  // every later point has D > T: no list can change
-          done = true;
-          break;
-        }
-        const int j = left ? __builtin_amdgcn_readlane(il, l - lw)
-                           : __builtin_amdgcn_readlane(ir, r - rw);
-        if (left) {
-          --l;
-          Dl = dl_of();
-        } else {
-          ++r;
-          Dr = dr_of();
-        }
-        if ((uint32_t)(j - flo) < (uint32_t)fspan) continue;  // the query's fold
-        js[u] = j;
-        dqs[u] = Dn;
-        cnt = u + 1;
+    for (int side = 0; side < 2; ++side) {
+      if (side == 0 ? !lact : !ract) continue;
+      const int p = side == 0 ? lo - 64 + lane : hi + lane;
+      const bool inr = p >= 0 && p < n;
+      const float kv = inr ? key[p] : 0.0f;
+      const int jv = inr ? idx[p] : 0;
+      const float tg = xg - kv;
+      const float Dv = tg * tg;
+      if (side == 0) {
+        lo -= 64;
+        lact = lo > 0;
+      } else {
+        hi += 64;
+        ract = hi < n;
       }
-      ts[u] = js[u] >= 0 ? Xl[(uint32_t)js[u] * ucpad] : 0.0f;
-    }
-    seen += cnt;
-    bool any = false;
+      if (!(lane_value(Dv, side == 0 ? 63 : 0) <= T)) {  // nearest point beyond T
+        if (side == 0) lact = false;
+        else ract = false;
+        continue;
+      }
+      const bool ok = inr && Dv <= T && (uint32_t)(jv - flo) >= (uint32_t)fspan;
+      unsigned long long mask = __ballot(ok);
+      seen += __popcll(mask);
+      bool changed = false;
+      while (mask) {
+        constexpr int BATCH = 8;
+        int js[BATCH];
+        float dqs[BATCH], ts[BATCH];
+        bool pass[BATCH];
+        int cnt = 0;
 #pragma unroll
-    for (int u = 0; u < BATCH; ++u) {
-      ts[u] = xq - ts[u];
-      pass[u] = u < cnt && fmaf(ts[u], ts[u], dqs[u]) < lim;
-      any |= pass[u];
-    }
-    if (__ballot(any)) {
+        for (int u = 0; u < BATCH; ++u) {
+          if (mask) {
+            const int b = __ffsll((long long)mask) - 1;
+            mask &= mask - 1;
+            js[u] = __builtin_amdgcn_readlane(jv, b);
+            dqs[u] = lane_value(Dv, b);
+            ts[u] = Xl[(uint32_t)js[u] * ucpad];
+            cnt = u + 1;
+          }
+        }
+        bool any = false;
 #pragma unroll
-      for (int u = 0; u < BATCH; ++u) {
-        if (pass[u]) {
-          const float dd = dqs[u] + ts[u] * ts[u];
-          if (pair_less(dd, js[u], Ld[K - 1], Li[K - 1])) {
-            topk_insert_pair<K>(Ld, Li, dd, js[u]);
-            lim = Ld[K - 1] < kMaxF ? fmaf(Ld[K - 1], kScreen, kTiny) : kInf;
+        for (int u = 0; u < BATCH; ++u) {
+          ts[u] = xq - ts[u];
+          pass[u] = u < cnt && fmaf(ts[u], ts[u], dqs[u]) < lim;
+          any |= pass[u];
+        }
+        if (__ballot(any)) {
+#pragma unroll
+          for (int u = 0; u < BATCH; ++u) {
+            if (pass[u]) {
+              const float dd = dqs[u] + ts[u] * ts[u];
+              if (pair_less(dd, js[u], Ld[K - 1], Li[K - 1])) {
+                topk_insert_pair<K>(Ld, Li, dd, js[u]);
+                lim = Ld[K - 1] < kMaxF ? fmaf(Ld[K - 1], kScreen, kTiny) : kInf;
+                changed = true;
+              }
+            }
           }
         }
       }
+      if (__ballot(changed)) T = wave_max(live ? Ld[K - 1] : -kInf);
     }
-    T = wave_max(live ? Ld[K - 1] : -kInf);
   }
   if (a.visited && lane == 0) {
     const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
