@@ -78,9 +78,12 @@ def list_size(kmax: int) -> int:
     return next(v for v in LIST_SIZES if kmax <= v)
 
 
-def traffic_per_launch(kernel_tag: str, workload: str):
-    """HBM bytes per launch from profiles/traffic_<kernel>_<workload>.md, or None."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel_tag}_{workload}.md")
+def traffic_per_launch(workload: str):
+    """HBM bytes per search launch from profiles/traffic_search_<workload>.md, or
+    None.  The summary averages over the pruned-search kernels the autotuner ran
+    in the counted selections (which of them it keeps is decided by timing, so
+    the summary is keyed by workload and names its kernels)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_search_{workload}.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", open(path).read())
@@ -186,9 +189,8 @@ def bench_single(args, hip, eng, form: int) -> dict:
         ms_dom = sum(x["ms_search"] for x in results)
     dom_evals = args.steps * sum((d - t) * n for t in range(first, steps_sel))
     launches = args.steps * (steps_sel - first)
-    tag = re.sub(r"[^A-Za-z0-9]+", "_", kernel).strip("_")
     rf = roofline(4.0 * n * dom_evals, ms_dom, launches, kernel,
-                  traffic_per_launch(tag, args.workload),
+                  traffic_per_launch(args.workload) if form == 1 else None,
                   "4*n bytes (one FP32 D row) per (query x candidate) evaluation")
 
     parity = None
@@ -272,7 +274,7 @@ def bench_batch(args, hip, eng, form: int) -> dict:
     family = "k_batch_pruned" if form == 1 else "k_batch_dense"
     kernel = f"{family}<K> (K = {', '.join(map(str, sizes))})"
     rf = roofline(dom_bytes, ms_dom, launches, kernel,
-                  traffic_per_launch(family, "c3"),
+                  traffic_per_launch("c3") if form == 1 else None,
                   "4*n_p bytes per (query x candidate) evaluation of problem p")
 
     # the comparison: 64 sequential plain-form selections of the same problems
@@ -329,15 +331,23 @@ def bench_gram(args, hip, eng) -> dict:
     hip.free(xp)
     hip.free(dp)
     flops = 2.0 * n * n * d  # algorithmic: one multiply-add per (pair, feature)
+    # products the matrix cores execute: the upper triangle of 128x128 tiles
+    # (diagonal tiles whole), features padded to 16, six split products each
+    nb = (n + 127) // 128
+    executed = 6 * 2.0 * (nb * (nb + 1) // 2) * 128 * 128 * ((d + 15) // 16 * 16)
     t_m, t_e = times["mfma"] / 1e3, times["exact"] / 1e3
     return {"gram": args.gram, "n": n, "d": d,
             "mfma_ms": times["mfma"], "exact_ms": times["exact"],
             "mfma_alg_TFLOPs": flops / t_m / 1e12,
             "mfma_util": 6 * flops / t_m / 2.5e15,  # six bf16 products per term
+            "mfma_executed_TFLOPs": executed / t_m / 1e12,
+            "mfma_pipe_util": executed / t_m / 2.5e15,
             "mfma_write_GBps": 4.0 * n * n / t_m / 1e9,
             "exact_alg_TFLOPs": flops / t_e / 1e12,
-            "note": "mfma_util counts the six split products against the 2.5 PFLOP/s dense "
-                    "BF16 peak; D write is 4n^2 bytes"}
+            "note": "mfma_util counts six split products per (pair, feature) of the full "
+                    "matrix against the 2.5 PFLOP/s dense BF16 peak; mfma_pipe_util counts "
+                    "the products executed (one triangle, mirrored); D write is 4n^2 bytes; "
+                    "times include the mean and split kernels"}
 
 
 def main() -> int:

@@ -6,9 +6,11 @@
 // exact remainders), and the dot product takes the six partial products down
 // to 2^-24 relative, hi.hi + hi.mid + mid.hi + mid.mid + hi.lo + lo.hi, each
 // a v_mfma_f32_32x32x16_bf16 with FP32 accumulation.  Norms come from the same
-// three-part values, summed in FP64.  The result is an approximation of the
-// exact distance within ~1e-6 relative on typical data (the bound the
-// accuracy test checks is 1e-5, against an FP64 oracle); it is not the
+// three-part values, summed in FP64 (an FP64 final combination was measured
+// 8% slower and no more accurate: the FP32 accumulation dominates the error).
+// The result is an approximation of the exact distance within ~1e-6 relative
+// on typical data, 6e-6 on 37 features with scales spanning four decades (the
+// bound the accuracy test checks is 1e-5, against an FP64 oracle); it is not the
 // reference's bit pattern, which only the sequential FP32 sum of k_gram_exact
 // reproduces, so the selection itself does not start from it.
 #include "common.h"
@@ -35,93 +37,124 @@ __global__ __launch_bounds__(256) void k_feature_mean(const float* __restrict__ 
   if (threadIdx.x == 0) mean[blockIdx.x] = s[0] / (double)n;
 }
 
-// Split of the centred values: S[p][i][f], p = hi, mid, lo, f padded to dp
-// with zeros; norm[i] = sum over f of (hi + mid + lo)^2 in FP64.  One block
-// per point.
+// Split of the centred values into three bf16 planes, stored pre-swizzled for
+// the MFMA operand fragments: for row block rb (32 points), k-step ks (16
+// features) and plane p, the 64 bf16x8 vectors a wave loads for one fragment
+// are contiguous -- vector (half * 32 + r) holds point 32*rb + r, features
+// 16*ks + 8*half .. +7 -- so a fragment is one coalesced 1 KB read:
+//   S8[((p * nrb + rb) * nks + ks) * 64 + half * 32 + r].
+// Rows past n and features past d are zero.  norm[i] = sum over f of
+// (hi + mid + lo)^2 in FP64, rounded to float.  One block per row block: 32
+// points x 8 groups of 8 features.
 __global__ __launch_bounds__(256) void k_split(const float* __restrict__ X, int64_t n, int d,
-                                               int dp, const double* __restrict__ mean,
-                                               __bf16* __restrict__ S, float* __restrict__ norm) {
-  __shared__ double s[256];
-  const int64_t i = blockIdx.x;
-  const size_t plane = (size_t)n * dp;
+                                               int nrb, int nks, const double* __restrict__ mean,
+                                               bf16x8* __restrict__ S8,
+                                               float* __restrict__ norm) {
+  __shared__ double s[8][32];
+  const int r = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const int rb = blockIdx.x;
+  const int64_t i = (int64_t)rb * 32 + r;
+  const size_t plane = (size_t)nrb * nks * 64;
   double acc = 0.0;
-  for (int f = threadIdx.x; f < dp; f += 256) {
-    __bf16 h = (__bf16)0.0f, m = (__bf16)0.0f, l = (__bf16)0.0f;
-    if (f < d) {
-      const float xc = (float)((double)X[i * d + f] - mean[f]);
-      h = (__bf16)xc;
-      const float r1 = xc - (float)h;
-      m = (__bf16)r1;
-      const float r2 = r1 - (float)m;
-      l = (__bf16)r2;
-      const double v = (double)(float)h + (double)(float)m + (double)(float)l;
+  for (int f0 = g * 8; f0 < nks * 16; f0 += 64) {
+    bf16x8 h, m, l;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int f = f0 + e;
+      float xc = 0.0f;
+      if (i < n && f < d) xc = (float)((double)X[i * d + f] - mean[f]);
+      const __bf16 hb = (__bf16)xc;
+      const float r1 = xc - (float)hb;
+      const __bf16 mb = (__bf16)r1;
+      const float r2 = r1 - (float)mb;
+      const __bf16 lb = (__bf16)r2;
+      h[e] = hb;
+      m[e] = mb;
+      l[e] = lb;
+      const double v = (double)(float)hb + (double)(float)mb + (double)(float)lb;
       acc += v * v;
     }
-    S[i * dp + f] = h;
-    S[plane + i * dp + f] = m;
-    S[2 * plane + i * dp + f] = l;
+    const size_t o = ((size_t)rb * nks + f0 / 16) * 64 + ((f0 / 8) & 1) * 32 + r;
+    S8[o] = h;
+    S8[plane + o] = m;
+    S8[2 * plane + o] = l;
   }
-  s[threadIdx.x] = acc;
+  s[g][r] = acc;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-    __syncthreads();
+  if (g == 0 && i < n) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) t += s[q][r];
+    norm[i] = (float)t;
   }
-  if (threadIdx.x == 0) norm[i] = (float)s[0];
 }
 
-// One 128x128 tile of D per block, a 64x64 quarter per wave held as 2x2
-// accumulators of v_mfma_f32_32x32x16_bf16 (each A and B fragment serves two
-// products).  Operand fragments: lane l holds row (l mod 32) of A and column
-// (l mod 32) of B at k = 8*(l/32) .. +7 -- 16 contiguous bytes of the
-// [n][dp] split arrays.  Accumulator element r of lane l is row
-// 8*(r/4) + 4*(l/32) + r%4, column l mod 32 (checked on the device against a
-// plain product).
-__global__ __launch_bounds__(256) void k_gram_mfma(const __bf16* __restrict__ S,
+// One 128x128 tile (bi, bj), bi <= bj, of the upper triangle per block; each
+// wave holds a 64x64 quarter as 2x2 accumulators of v_mfma_f32_32x32x16_bf16
+// (each A and B fragment serves two products).  Fragment of lane l: row (l mod
+// 32) of A, column (l mod 32) of B, k = 8*(l/32) .. +7 -- one bf16x8 of S8.
+// Three waves per SIMD (93 VGPRs, 64 accumulation registers) overlap one
+// wave's fragment loads with another's products.
+// Accumulator element e of lane l is row 8*(e/4) + 4*(l/32) + e%4, column
+// l mod 32.  The tile is stored twice, as D[row][col] and, four rows at a time
+// as float4, as D[col][row]: D is exactly symmetric and the matrix cores do
+// half the work.  A diagonal tile stores only row < col (and 0 on the
+// diagonal).  Blocks map to XCDs in contiguous ranges of the column-major
+// triangle order, so each XCD's L2 serves consecutive tiles of one B panel.
+__global__ __launch_bounds__(256) void k_gram_mfma(const bf16x8* __restrict__ S8,
                                                    const float* __restrict__ norm, int64_t n,
-                                                   int dp, float* __restrict__ D) {
+                                                   int nrb, int nks, int64_t ntiles,
+                                                   float* __restrict__ D) {
+  const int64_t per_xcd = (ntiles + 7) / 8;
+  const int64_t t = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (t >= ntiles) return;
+  // column-major upper triangle: tile column bj holds tiles bi = 0..bj
+  int64_t bj = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while (bj * (bj + 1) / 2 > t) --bj;
+  while ((bj + 1) * (bj + 2) / 2 <= t) ++bj;
+  const int64_t bi = t - bj * (bj + 1) / 2;
+
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int64_t i0 = (int64_t)blockIdx.y * 128 + (wave >> 1) * 64;
-  const int64_t j0 = (int64_t)blockIdx.x * 128 + (wave & 1) * 64;
-  const size_t pstep = (size_t)n * dp / 8;  // one split plane in bf16x8 units (dp % 16 == 0)
-  const bf16x8 zero = {};
+  const int64_t i0 = bi * 128 + (wave >> 1) * 64;
+  const int64_t j0 = bj * 128 + (wave & 1) * 64;
+  const size_t pstep = (size_t)nrb * nks * 64;
   const bf16x8* A[2];
   const bf16x8* B[2];
-  bool va[2], vb[2];
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int64_t ri = i0 + 32 * t + (lane & 31), rj = j0 + 32 * t + (lane & 31);
-    va[t] = ri < n;
-    vb[t] = rj < n;
-    A[t] = reinterpret_cast<const bf16x8*>(S + (va[t] ? ri : 0) * dp + (lane >> 5) * 8);
-    B[t] = reinterpret_cast<const bf16x8*>(S + (vb[t] ? rj : 0) * dp + (lane >> 5) * 8);
+  for (int u = 0; u < 2; ++u) {
+    A[u] = S8 + (size_t)(i0 / 32 + u) * nks * 64 + lane;
+    B[u] = S8 + (size_t)(j0 / 32 + u) * nks * 64 + lane;
   }
   floatx16 acc[2][2] = {};
-  for (int k = 0; k < dp; k += 16) {
-    const int kv = k / 8;
-    bf16x8 a[2][3], b[2][3];
+  for (int ks = 0; ks < nks; ++ks) {
+    const size_t ko = (size_t)ks * 64;
+    bf16x8 fa[2][3], fb[2][3];  // [tile][plane]
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
-        a[t][p] = va[t] ? A[t][p * pstep + kv] : zero;
-        b[t][p] = vb[t] ? B[t][p * pstep + kv] : zero;
+        fa[u][p] = A[u][p * pstep + ko];
+        fb[u][p] = B[u][p * pstep + ko];
       }
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
       for (int tj = 0; tj < 2; ++tj) {
+        const bf16x8* x = fa[ti];
+        const bf16x8* y = fb[tj];
         floatx16 c = acc[ti][tj];  // small terms first
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ti][0], b[tj][2], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ti][2], b[tj][0], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ti][1], b[tj][1], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ti][0], b[tj][1], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ti][1], b[tj][0], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ti][0], b[tj][0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[0], y[2], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[2], y[0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[1], y[1], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[0], y[1], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[1], y[0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[0], y[0], c, 0, 0, 0);
         acc[ti][tj] = c;
       }
   }
+  const bool diag = bi == bj;
+  const bool vec = (n & 3) == 0;  // float4 mirror stores stay aligned
 #pragma unroll
   for (int tj = 0; tj < 2; ++tj) {
     const int64_t col = j0 + 32 * tj + (lane & 31);
@@ -130,36 +163,60 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const __bf16* __restrict__ S,
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int64_t row = i0 + 32 * ti + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
-        if (row < n) {
-          float v = (norm[row] + nc) - 2.0f * acc[ti][tj][r];
-          v = v < 0.0f ? 0.0f : v;
-          D[row * n + col] = row == col ? 0.0f : v;
+      for (int g = 0; g < 4; ++g) {
+        const int64_t row0 = i0 + 32 * ti + 8 * g + 4 * (lane >> 5);
+        float v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int64_t row = row0 + e;
+          const float nr = row < n ? norm[row] : 0.0f;
+          const float x = (nr + nc) - 2.0f * acc[ti][tj][4 * g + e];
+          v[e] = row == col ? 0.0f : (x < 0.0f ? 0.0f : x);
+        }
+        if (!diag && vec && row0 + 3 < n) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) D[(row0 + e) * n + col] = v[e];
+          *reinterpret_cast<float4*>(D + col * n + row0) = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int64_t row = row0 + e;
+            if (row >= n || (diag && row > col)) continue;
+            D[row * n + col] = v[e];
+            D[col * n + row] = v[e];
+          }
         }
       }
   }
 }
 
+static int gram_blocks_padded(int64_t n) { return (int)((n + 127) / 128); }
+
 size_t gram_mfma_scratch_bytes(int64_t n, int d) {
-  const int dp = (d + 15) / 16 * 16;
-  return 3 * (size_t)n * dp * sizeof(__bf16) + (size_t)n * sizeof(float) +
+  const int nks = (d + 15) / 16;
+  const size_t nrb = (size_t)gram_blocks_padded(n) * 4;
+  return 3 * nrb * nks * 64 * sizeof(bf16x8) + ((size_t)n * sizeof(float) + 255) / 256 * 256 +
          (size_t)d * sizeof(double) + 256;
 }
 
 hipError_t launch_gram_mfma(const float* X, const float* XT, int64_t n, int d, float* D,
                             void* scratch, hipStream_t s) {
-  const int dp = (d + 15) / 16 * 16;
+  const int nks = (d + 15) / 16;
+  const int nbt = gram_blocks_padded(n);
+  const int nrb = nbt * 4;
   char* p = static_cast<char*>(scratch);
-  __bf16* S = reinterpret_cast<__bf16*>(p);
-  p += 3 * (size_t)n * dp * sizeof(__bf16);
+  bf16x8* S8 = reinterpret_cast<bf16x8*>(p);
+  p += 3 * (size_t)nrb * nks * 64 * sizeof(bf16x8);
   float* norm = reinterpret_cast<float*>(p);
   p += ((size_t)n * sizeof(float) + 255) / 256 * 256;
   double* mean = reinterpret_cast<double*>(p);
   hipLaunchKernelGGL(k_feature_mean, dim3(d), dim3(256), 0, s, XT, n, mean);
-  hipLaunchKernelGGL(k_split, dim3((unsigned)n), dim3(256), 0, s, X, n, d, dp, mean, S, norm);
-  const dim3 g((unsigned)((n + 127) / 128), (unsigned)((n + 127) / 128));
-  hipLaunchKernelGGL(k_gram_mfma, g, dim3(256), 0, s, S, norm, n, dp, D);
+  hipLaunchKernelGGL(k_split, dim3((unsigned)nrb), dim3(256), 0, s, X, n, d, nrb, nks, mean, S8,
+                     norm);
+  const int64_t ntiles = (int64_t)nbt * (nbt + 1) / 2;
+  const int64_t grid = (ntiles + 7) / 8 * 8;
+  hipLaunchKernelGGL(k_gram_mfma, dim3((unsigned)grid), dim3(256), 0, s, S8, norm, n, nrb, nks,
+                     ntiles, D);
   return hipGetLastError();
 }
 

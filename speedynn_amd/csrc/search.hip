@@ -275,6 +275,80 @@ __device__ __forceinline__ float seed_bound(const SearchProblem& a, const float*
   return live ? B[K - 1] : -kInf;
 }
 
+// Top-K insertion ordered by (distance, index) whatever the offering order:
+// the list ends as the K smallest pairs, which is what the ascending-index
+// scan with strict comparisons produces.
+__device__ __forceinline__ bool pair_less(float a, int ja, float b, int jb) {
+  return a < b || (a == b && ja < jb);
+}
+
+template <int K>
+__device__ __forceinline__ void topk_insert_pair(float (&Ld)[K], int (&Li)[K], float dd, int j) {
+#pragma unroll
+  for (int m = K - 1; m > 0; --m) {
+    const bool cp = pair_less(dd, j, Ld[m - 1], Li[m - 1]);
+    const bool ch = pair_less(dd, j, Ld[m], Li[m]);
+    const float nd = cp ? Ld[m - 1] : (ch ? dd : Ld[m]);
+    const int ni = cp ? Li[m - 1] : (ch ? j : Li[m]);
+    Ld[m] = nd;
+    Li[m] = ni;
+  }
+  if (pair_less(dd, j, Ld[0], Li[0])) {
+    Ld[0] = dd;
+    Li[0] = j;
+  }
+}
+
+// Pending insertions of one lane: up to pend_slots<K>() (distance, index)
+// pairs that passed the exact test against the lane's list as it stood, held
+// in the wave's LDS area (slot s of lane l at word s*64 + l, distances then
+// indices), so the K-slot insertion network runs once per pending entry of
+// the busiest lane at a flush instead of once per point on which any of the
+// 64 lanes inserts.  A flush inserts a lane's entries in arrival order, which
+// is ascending point index.  A visit batch pushes at most 8 entries per lane,
+// so a flush comes when a lane holds more than slots - 8: 12 slots where the
+// list length already limits the kernel to six waves per SIMD (K >= 15), 10
+// below, so that LDS still admits eight 4-wave blocks per CU.
+template <int K>
+constexpr int pend_slots() {
+  return K >= 15 ? 12 : 10;
+}
+
+// Words of LDS one wave of the per-wave pruned kernels uses: the pending
+// buffer during the scan, the epilogue labels (K slots x 64 lanes) after it.
+template <int K>
+constexpr int wave_lds_words() {
+  return 64 * (K > 2 * pend_slots<K>() ? K : 2 * pend_slots<K>());
+}
+
+// Flush of the pending buffer `buf` (the lane's slot 0; `pc` entries): insert
+// the entries in arrival order, then tighten the lane's screen limit (forward:
+// widened for the fma screen; backward: exact) and its bound thr, and return
+// the wave's bound, the largest thr.  `top` is the wave's largest pc.
+template <int K, int MULT>
+__device__ __forceinline__ float flush_pending(const float* buf, int& pc, float (&Ld)[K],
+                                               int (&Li)[K], float& lim, float& thr, float T,
+                                               bool live) {
+  constexpr float kScreen = 1.00000095367431640625f;  // 1 + 2^-20
+  constexpr float kTiny = 1.0e-36f;
+  int top = pc;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) top = max(top, __shfl_xor(top, o));
+  for (int sl = 0; sl < top; ++sl) {
+    if (sl < pc) {
+      const float dd = buf[sl * 64];
+      const int jj = __float_as_int(buf[(pend_slots<K>() + sl) * 64]);
+      if (dd < Ld[K - 1]) topk_insert<K>(Ld, Li, dd, jj);
+    }
+  }
+  pc = 0;
+  if (top == 0) return T;
+  if (Ld[K - 1] < kMaxF)
+    lim = fminf(lim, MULT > 0 ? fmaf(Ld[K - 1], kScreen, kTiny) : Ld[K - 1]);
+  thr = live ? fminf(thr, Ld[K - 1]) : -__int_as_float(0x7f800000);
+  return wave_max(thr);
+}
+
 // Pruned search of one (query q, candidate block cb) item by one wave
 // (precision form: forward steps after the first, every backward step).
 //
@@ -299,9 +373,15 @@ __device__ __forceinline__ float seed_bound(const SearchProblem& a, const float*
 // range): |fl(D+fl(t*t)) - fma(t,t,D)| <= 3.01*2^-24*(D+t*t) for D, t*t >= 0,
 // so a rejected point cannot pass the exact test; passes recompute d in the
 // reference's order.  Backward tests are exact (no screen: the fma error is
-// not bounded relative to D - t^2).  No barriers.
+// not bounded relative to D - t^2).  Points that beat the list's last entry
+// go to the lane's pending buffer (pend_slots<K>() slots in `buf`, the wave's
+// LDS area offset by the lane), flushed into the list when a lane's buffer
+// may overflow and at the end; the bound and screen limit tighten at each
+// flush.
+// No barriers.  Registers decide the speed: the search is latency-bound, and
+// at K = 20 this body fits six waves per SIMD.
 template <int K, int MULT>
-__device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* lab, int64_t q,
+__device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, int64_t q,
                                             int cb) {
   if (a.st->stopped) return;
   const int ncand = a.st->ncand;
@@ -319,8 +399,6 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* lab, 
   const int64_t flo = (q / npf) * npf;
   const int64_t fhi = flo + npf < n ? flo + npf : n;
   const float kInf = __int_as_float(0x7f800000);
-  constexpr float kScreen = 1.00000095367431640625f;  // 1 + 2^-20
-  constexpr float kTiny = 1.0e-36f;
   auto dist = [](float dq, float p) { return MULT > 0 ? dq + p : dq - p; };
 
   float thr = seed_bound<K, MULT>(a, Xl, Dq, xq, q, cpad, live);
@@ -337,6 +415,9 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* lab, 
   // padding lanes never pass (-inf)
   float lim = screen_limit<MULT>(thr);
   unsigned long long seen = 0;
+  int pc = 0;  // this lane's pending entries
+
+  auto flush = [&]() { T = flush_pending<K, MULT>(buf, pc, Ld, Li, lim, thr, T, live); };
 
   // D (and M) of 64 points per chunk; a window of 4 chunks is in flight.  The
   // rows stream through once (non-temporal), leaving the caches to Xc.
@@ -365,7 +446,6 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* lab, 
     unsigned long long mask = __ballot(ok);
     if (!mask) continue;
     seen += __popcll(mask);
-    bool changed = false;
     while (mask) {
       constexpr int BATCH = 8;
       int js[BATCH];
@@ -383,7 +463,7 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* lab, 
           cnt = u + 1;
         }
       }
-      // screen the batch; the exact path runs only if some lane may pass
+      // screen the batch; the exact test runs only if some lane may pass
       bool any = false;
 #pragma unroll
       for (int u = 0; u < BATCH; ++u) {
@@ -396,53 +476,24 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* lab, 
 #pragma unroll
         for (int u = 0; u < BATCH; ++u) {
           if (pass[u]) {
-            const float p = ts[u] * ts[u];
-            const float dd = dist(dqs[u], p);
+            const float dd = dist(dqs[u], ts[u] * ts[u]);
             if (dd < Ld[K - 1]) {
-              topk_insert<K>(Ld, Li, dd, js[u]);
-              lim = fminf(lim, MULT > 0 ? (Ld[K - 1] < kMaxF ? fmaf(Ld[K - 1], kScreen, kTiny)
-                                                              : kInf)
-                                        : Ld[K - 1]);
-              changed = true;
+              buf[pc * 64] = dd;
+              buf[(pend_slots<K>() + pc) * 64] = __int_as_float(js[u]);
+              ++pc;
             }
           }
         }
+        if (__ballot(pc > pend_slots<K>() - BATCH)) flush();
       }
     }
-    if (__ballot(changed)) {
-      thr = live ? fminf(thr, Ld[K - 1]) : -kInf;
-      T = wave_max(thr);
-    }
   }
+  flush();
   if (a.visited && lane == 0) {
     const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
     atomicAdd(a.visited, seen * (unsigned long long)nlive);  // point evaluations performed
   }
-  finish_query<K>(a, Li, lab, q, c, live);
-}
-
-// Top-K insertion ordered by (distance, index) whatever the offering order:
-// the list ends as the K smallest pairs, which is what the ascending-index
-// scan with strict comparisons produces.
-__device__ __forceinline__ bool pair_less(float a, int ja, float b, int jb) {
-  return a < b || (a == b && ja < jb);
-}
-
-template <int K>
-__device__ __forceinline__ void topk_insert_pair(float (&Ld)[K], int (&Li)[K], float dd, int j) {
-#pragma unroll
-  for (int m = K - 1; m > 0; --m) {
-    const bool cp = pair_less(dd, j, Ld[m - 1], Li[m - 1]);
-    const bool ch = pair_less(dd, j, Ld[m], Li[m]);
-    const float nd = cp ? Ld[m - 1] : (ch ? dd : Ld[m]);
-    const int ni = cp ? Li[m - 1] : (ch ? j : Li[m]);
-    Ld[m] = nd;
-    Li[m] = ni;
-  }
-  if (pair_less(dd, j, Ld[0], Li[0])) {
-    Ld[0] = dd;
-    Li[0] = j;
-  }
+  finish_query<K>(a, Li, buf, q, c, live);
 }
 
 // First step of a forward selection (D = 0) over sorted columns: one lane per
@@ -526,8 +577,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_search_dense(const SearchProblem
 // nblk blocks of one query are consecutive waves that read the query's D (and
 // M) row once into L2.
 template <int K, int MULT>
-__global__ __launch_bounds__(256) void k_search_pruned(const SearchProblem a, int cb_major) {
-  __shared__ float lds[4 * K * 64];  // epilogue labels
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(K <= 20 && MULT > 0 ? 6 : 1)))  // forward: <= 80 VGPRs
+void k_search_pruned(const SearchProblem a, int cb_major) {
+  __shared__ float lds[4 * wave_lds_words<K>()];  // pending buffers, then epilogue labels
   const int wave = uniform(threadIdx.x >> 6);
   const int nblk = a.cpad / 64;
   const int64_t per_xcd = (a.n + 7) / 8;
@@ -536,13 +589,13 @@ __global__ __launch_bounds__(256) void k_search_pruned(const SearchProblem a, in
   const int cb = (int)(cb_major ? local / per_xcd : local % nblk);
   const int64_t q = (int64_t)(blockIdx.x % 8) * per_xcd + qi;
   if (q >= a.n || cb >= nblk || qi >= per_xcd) return;  // whole wave; no barriers
-  pruned_wave<K, MULT>(a, lds + wave * (K * 64) + (threadIdx.x & 63), q, cb);
+  pruned_wave<K, MULT>(a, lds + wave * wave_lds_words<K>() + (threadIdx.x & 63), q, cb);
 }
 
 // Pruned search with the row scan shared by the candidate blocks of a query:
 // one workgroup per query, wave w owns candidate block w (waves past the
-// last block only help scanning).  Bounds and the exact path are those of
-// pruned_wave.  Per chunk of R*64*W points each lane tests R points against
+// last block only help scanning).  Bounds, the exact path and the pending
+// buffers (one LDS area per wave) are those of pruned_wave.  Per chunk of R*64*W points each lane tests R points against
 // the largest bound of the workgroup's waves; the passing points go, in
 // ascending index order, to a queue in LDS (index, D, lower bound), and every
 // wave visits the queue, skipping entries above its own bound.  The row is
@@ -558,6 +611,7 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
   constexpr int LDS_Q = 3 * CH;       // queue: index, D, lower bound
   constexpr int LDS_E = 4 * K * 64;   // epilogue labels, 4 waves at a time
   __shared__ float lds[LDS_Q > LDS_E ? LDS_Q : LDS_E];
+  __shared__ float pend[W * 2 * pend_slots<K>() * 64];  // pending buffers, one area per wave
   __shared__ float Tw[W];
   __shared__ int wcount[R * W];
   int* qj = reinterpret_cast<int*>(lds);
@@ -587,8 +641,6 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
   const float* __restrict__ Mq = MULT > 0 ? nullptr : a.M + (int64_t)q * n;
   const float* __restrict__ Xl = a.Xc + (active ? cb * 64 : 0) + lane;
   const float kInf = __int_as_float(0x7f800000);
-  constexpr float kScreen = 1.00000095367431640625f;  // 1 + 2^-20
-  constexpr float kTiny = 1.0e-36f;
   auto dist = [](float dq, float p) { return MULT > 0 ? dq + p : dq - p; };
   const float xq = active ? Xl[(int64_t)q * cpad] : 0.0f;
 
@@ -605,6 +657,12 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
   }
   float lim = screen_limit<MULT>(thr);  // as in pruned_wave
   unsigned long long seen = 0;
+  float* buf = pend + wave * (2 * pend_slots<K>() * 64) + lane;
+  int pc = 0;  // this lane's pending entries
+  auto flush = [&]() {
+    T = flush_pending<K, MULT>(buf, pc, Ld, Li, lim, thr, T, live);
+    if (lane == 0) Tw[wave] = T;
+  };
 
   auto ldD = [&](int j) { return j < n ? __builtin_nontemporal_load(Dq + j) : kInf; };
   auto ldM = [&](int j) { return (MULT < 0 && j < n) ? __builtin_nontemporal_load(Mq + j) : 0.0f; };
@@ -672,7 +730,6 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
       unsigned long long mask = __ballot(has && lbe <= T);
       if (!mask) continue;
       seen += __popcll(mask);
-      bool changed = false;
       while (mask) {
         constexpr int BATCH = 8;
         int js[BATCH];
@@ -704,23 +761,18 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
             if (pass[u]) {
               const float dd = dist(dqs[u], ts[u] * ts[u]);
               if (dd < Ld[K - 1]) {
-                topk_insert<K>(Ld, Li, dd, js[u]);
-                lim = fminf(lim, MULT > 0 ? (Ld[K - 1] < kMaxF ? fmaf(Ld[K - 1], kScreen, kTiny)
-                                                                : kInf)
-                                          : Ld[K - 1]);
-                changed = true;
+                buf[pc * 64] = dd;
+                buf[(pend_slots<K>() + pc) * 64] = __int_as_float(js[u]);
+                ++pc;
               }
             }
           }
+          if (__ballot(pc > pend_slots<K>() - BATCH)) flush();
         }
-      }
-      if (__ballot(changed)) {
-        thr = live ? fminf(thr, Ld[K - 1]) : -kInf;
-        T = wave_max(thr);
-        if (lane == 0) Tw[wave] = T;
       }
     }
   }
+  flush();
   if (a.visited && lane == 0 && active) {
     const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
     atomicAdd(a.visited, seen * (unsigned long long)nlive);
@@ -746,7 +798,7 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
 template <int K>
 __global__ __launch_bounds__(256) void k_search_walk(const SearchProblem a,
                                                      const int* __restrict__ rank) {
-  __shared__ float lds[4 * K * 64];
+  __shared__ float lds[4 * wave_lds_words<K>()];
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int nblk = a.cpad / 64;
@@ -756,7 +808,7 @@ __global__ __launch_bounds__(256) void k_search_walk(const SearchProblem a,
   const int cb = (int)(local / per_xcd);
   const int64_t q = (int64_t)(blockIdx.x % 8) * per_xcd + qi;
   if (q >= a.n || cb >= nblk) return;  // whole wave; no barriers
-  float* lab = lds + wave * (K * 64) + lane;
+  float* lab = lds + wave * wave_lds_words<K>() + lane;
   const SelState* st = a.st;
   if (st->stopped) return;
   if (st->nonfinite) {
@@ -944,10 +996,10 @@ template <int K, int MULT>
 __global__ __launch_bounds__(256) void k_batch_pruned(const SearchProblem* __restrict__ probs,
                                                       const int64_t* __restrict__ start, int P,
                                                       unsigned long long* __restrict__ counter) {
-  __shared__ float lds[4 * K * 64];
+  __shared__ float lds[4 * wave_lds_words<K>()];
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  float* lab = lds + wave * (K * 64) + lane;
+  float* lab = lds + wave * wave_lds_words<K>() + lane;
   const int64_t total = start[P];
   for (;;) {
     unsigned long long it = 0;

@@ -204,11 +204,14 @@ def test_compensated_precision_selects_the_same(engine, name):
     assert_same_result(got, expected_arrays(rec), name)
 
 
-def test_mfma_gram_within_1e5_of_fp64(engine):
+@pytest.mark.parametrize("n,d", [(1500, 300), (1001, 37)])
+def test_mfma_gram_within_1e5_of_fp64(engine, n, d):
     """(a) on the matrix cores: the BF16x3 MFMA Gram (SNN_FORM_MFMA) against
     the FP64 oracle, on unit-scale Gaussian features and on features with
-    scales 10^U(-2,2).  Tolerance 1e-5 relative; the diagonal is exactly 0."""
-    n, d = 1500, 300
+    scales 10^U(-2,2).  Tolerance 1e-5 relative; the diagonal is exactly 0 and
+    the matrix exactly symmetric.  Neither n is a multiple of the 128-point
+    tile (partial tiles); n = 1001 is odd, so every mirror store takes the
+    scalar path, and d = 37 pads the last 16-feature step with zeros."""
     Z = ds.gaussian_matrix(91, n, d)
     scales = ds.exp2(np.log2(10.0) * (4.0 * ds.uniforms(92, 0, d) - 2.0))
     cases = {"unit": Z, "scaled": (Z.astype(np.float64) * scales[None, :]).astype(np.float32)}
@@ -224,5 +227,9 @@ def test_mfma_gram_within_1e5_of_fp64(engine):
             rel = np.abs(got[r][off].astype(np.float64) - ref[off]) / ref[off]
             worst = max(worst, float(rel.max()))
             assert got[r][q] == 0.0
+        # one triangle is computed and mirrored: D is exactly symmetric
+        for r, q in enumerate(rows):
+            for r2, q2 in enumerate(rows):
+                assert got[r][q2] == got[r2][q], (name, q, q2)
         print(f"mfma gram {name}: max relative error {worst:.3e}", flush=True)
         assert worst <= 1e-5, (name, worst)
