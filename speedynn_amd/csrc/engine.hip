@@ -125,10 +125,13 @@ double elapsed(hipEvent_t a, hipEvent_t b) {
 // shape (n, d, list size, direction) and step range [2^b, 2^(b+1)) in this
 // process runs every applicable kernel on the same step, timed with HIP
 // events (their results are bit-identical; only the last run counts visits),
-// and the fastest serves the rest of that range.  SPEEDYNN_TUNING=off keeps
-// the rule by direction.
+// and the fastest serves the rest of that range.  A kernel that took more
+// than twice the winner's time is not tried again on that shape's later
+// ranges (the per-wave kernel, which applies everywhere, always is).
+// SPEEDYNN_TUNING=off keeps the rule by direction.
 std::mutex g_tune_mu;
 std::map<std::tuple<int64_t, int, int, int, int>, int> g_tuned;
+std::map<std::tuple<int64_t, int, int, int>, int> g_slow;  // bit v: variant v lost by 2x
 
 // Variant 4: the walk along the first chosen feature's sorted column, for a
 // forward selection's second step (rank != null); it has a step range of its own.
@@ -168,11 +171,26 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int
     *used |= 1 << (v - 1);
     return SNN_OK;
   }
+  const auto shape = std::make_tuple(sl.n, d, snn::search_list_size(sl.kmax), sl.mult);
   if (!off && nc > 1) {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tuned.find(key);
-    if (it != g_tuned.end()) variant = it->second;
-    else tune = true;
+    if (it != g_tuned.end()) {
+      variant = it->second;
+    } else {
+      auto sw = g_slow.find(shape);
+      const int slow = sw == g_slow.end() ? 0 : sw->second;
+      int kept = 0;
+      for (int i = 0; i < nc; ++i)
+        if (cands[i] == 1 || cands[i] == 4 || !(slow & (1 << cands[i]))) cands[kept++] = cands[i];
+      nc = kept;
+      if (nc > 1) {
+        tune = true;
+      } else {
+        variant = cands[0];
+        g_tuned[key] = variant;
+      }
+    }
   }
   if (!tune) {
     HIPCHK(launch_variant(sl, variant, rank, s));
@@ -194,17 +212,21 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int
   }
   HIPCHK(hipEventSynchronize(ev[nc]));
   variant = cands[0];
-  double best = elapsed(ev[0], ev[1]);
+  double t[4];
+  double best = t[0] = elapsed(ev[0], ev[1]);
   for (int i = 1; i < nc; ++i) {
-    const double t = elapsed(ev[i], ev[i + 1]);
-    if (t < best) {
-      best = t;
+    t[i] = elapsed(ev[i], ev[i + 1]);
+    if (t[i] < best) {
+      best = t[i];
       variant = cands[i];
     }
   }
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     g_tuned[key] = variant;
+    int& slow = g_slow[shape];
+    for (int i = 0; i < nc; ++i)
+      if (t[i] > 2.0 * best) slow |= 1 << cands[i];
   }
   return SNN_OK;
 }

@@ -7,7 +7,7 @@
 // to 2^-24 relative, hi.hi + hi.mid + mid.hi + mid.mid + hi.lo + lo.hi, each
 // a v_mfma_f32_32x32x16_bf16 with FP32 accumulation.  Norms come from the same
 // three-part values, summed in FP64 (an FP64 final combination was measured
-// 8% slower and no more accurate: the FP32 accumulation dominates the error).
+// no more accurate: the FP32 accumulation dominates the error).
 // The result is an approximation of the exact distance within ~1e-6 relative
 // on typical data, 6e-6 on 37 features with scales spanning four decades (the
 // bound the accuracy test checks is 1e-5, against an FP64 oracle); it is not the

@@ -12,6 +12,8 @@
 // Arithmetic of the plain paths is IEEE binary32 in the reference's operation
 // order; the build uses -ffp-contract=off, so a*b+c is never fused unless a
 // kernel calls fmaf() on purpose.
+#include <type_traits>
+
 #include "common.h"
 #include "kernels.h"
 #include "qsort_order.h"
@@ -448,64 +450,89 @@ __device__ float loss_rmse_norm(const float* __restrict__ y, const float* __rest
 // points: kLossLoaders loader waves read the tile's rows coalesced (256 bytes
 // per row), evaluate the terms and store them transposed in LDS; the last
 // wave of the block only adds, lane c along row c of the tile, while the
-// loaders work on the next tile (double-buffered, one barrier per tile).
-// term(q, p_q) gives the addend of point q from the prediction p_q.
+// loaders stage the next tile (double-buffered, one barrier per tile) and
+// fetch the one kLossAhead - 1 tiles further on.
+// term(y_q, p_q) gives the addend of point q from its label and prediction;
+// both are loaded kLossAhead - 1 tiles ahead.
 constexpr int kLossTile = 64;
 constexpr int kLossLoaders = 8;
 constexpr int kLossRows = 64 / kLossLoaders;   // rows per loader wave
 constexpr int kLossStride = kLossTile + 4;     // 16-byte aligned rows, distinct banks
 constexpr int kLossThreads = (kLossLoaders + 1) * 64;
+constexpr int kLossAhead = 4;                  // tiles a loader holds in registers
 constexpr int kLossLdsFloats = 2 * 64 * kLossStride;
 
 template <bool REV, class Term>
-__device__ __forceinline__ float lane_seq_sum(const float* __restrict__ P0, int64_t n,
+__device__ __forceinline__ float lane_seq_sum(const float* __restrict__ P0,
+                                              const float* __restrict__ y, int64_t n,
                                               float* __restrict__ tiles, Term term) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t ntile = (n + kLossTile - 1) / kLossTile;
   float s = 0.0f;
-  float raw[kLossRows];
-  auto fetch = [&](int64_t t) {  // loader: its rows of tile t
+  // loaders keep tile u in register slot u % kLossAhead: the loads of a tile
+  // are issued kLossAhead - 1 tiles before it is staged, which hides the
+  // memory latency behind the adder's work on the tiles in between
+  float raw[kLossAhead][kLossRows];
+  float ys[kLossAhead];  // the label of the lane's point, loaded with its rows
+  auto fetch = [&](int64_t t, float (&r_)[kLossRows], float& yv) {  // loader: its rows of tile t
     const int64_t k = t * kLossTile + lane;  // k-th point in summation order
     const int64_t q = REV ? n - 1 - k : k;
     const float* base = P0 + (int64_t)(wave * kLossRows) * n + q;
 #pragma unroll
-    for (int r = 0; r < kLossRows; ++r) raw[r] = k < n ? base[(int64_t)r * n] : 0.0f;
+    for (int r = 0; r < kLossRows; ++r) r_[r] = k < n ? base[(int64_t)r * n] : 0.0f;
+    yv = k < n ? y[q] : 0.0f;
   };
-  auto stage = [&](int64_t t, float* buf) {
+  auto stage = [&](int64_t t, const float (&r_)[kLossRows], float yv, float* buf) {
     const int64_t k = t * kLossTile + lane;
-    const int64_t q = REV ? n - 1 - k : k;
 #pragma unroll
     for (int r = 0; r < kLossRows; ++r)
-      buf[(wave * kLossRows + r) * kLossStride + lane] = k < n ? term(q, raw[r]) : 0.0f;
+      buf[(wave * kLossRows + r) * kLossStride + lane] = k < n ? term(yv, r_[r]) : 0.0f;
   };
   if (wave < kLossLoaders) {
-    fetch(0);
-    stage(0, tiles);
-    if (ntile > 1) fetch(1);
+#pragma unroll
+    for (int u = 0; u < kLossAhead; ++u)
+      if (u < ntile) fetch(u, raw[u], ys[u]);
+    stage(0, raw[0], ys[0], tiles);
   }
   __syncthreads();
-  for (int64_t t = 0; t < ntile; ++t) {
+  // one tile: slot U = t % kLossAhead is a compile-time index of raw
+  auto tile_step = [&](int64_t t, auto slot) {
+    constexpr int U = decltype(slot)::value;
+    if (t >= ntile) return;  // the same for every wave of the block
     const float* cur = tiles + (t & 1) * 64 * kLossStride;
     if (wave == kLossLoaders) {
       const float* row = cur + lane * kLossStride;
       if (n - t * kLossTile >= kLossTile) {
+        // all of the row's reads first, then the dependent adds: the LDS
+        // latency is paid once per tile instead of once per four terms
+        float4 v[kLossTile / 4];
 #pragma unroll
-        for (int i = 0; i < kLossTile; i += 4) {
-          const float4 v = *reinterpret_cast<const float4*>(row + i);
-          s = s + v.x;
-          s = s + v.y;
-          s = s + v.z;
-          s = s + v.w;
+        for (int i = 0; i < kLossTile / 4; ++i) v[i] = *reinterpret_cast<const float4*>(row + 4 * i);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < kLossTile / 4; ++i) {
+          s = s + v[i].x;
+          s = s + v[i].y;
+          s = s + v[i].z;
+          s = s + v[i].w;
         }
       } else {
         const int cnt = (int)(n - t * kLossTile);
         for (int i = 0; i < cnt; ++i) s = s + row[i];
       }
     } else if (t + 1 < ntile) {
-      stage(t + 1, tiles + ((t + 1) & 1) * 64 * kLossStride);
-      if (t + 2 < ntile) fetch(t + 2);
+      stage(t + 1, raw[(U + 1) % kLossAhead], ys[(U + 1) % kLossAhead],
+            tiles + ((t + 1) & 1) * 64 * kLossStride);
+      if (t + kLossAhead < ntile) fetch(t + kLossAhead, raw[U], ys[U]);  // tile t's slot is free
     }
     __syncthreads();  // tile t consumed, tile t + 1 staged
+  };
+  static_assert(kLossAhead == 4, "tile_step is unrolled for four slots");
+  for (int64_t t0 = 0; t0 < ntile; t0 += kLossAhead) {
+    tile_step(t0, std::integral_constant<int, 0>{});
+    tile_step(t0 + 1, std::integral_constant<int, 1>{});
+    tile_step(t0 + 2, std::integral_constant<int, 2>{});
+    tile_step(t0 + 3, std::integral_constant<int, 3>{});
   }
   return s;
 }
@@ -522,14 +549,14 @@ __device__ float lane_loss_value(int code, const float* __restrict__ y,
                                  float* __restrict__ tile) {
   const float* __restrict__ p = P0 + (int64_t)(threadIdx.x & 63) * n;
   auto rmse = [&]() {
-    const float s = lane_seq_sum<false>(P0, n, tile, [&](int64_t q, float pq) {
-      const float t = y[q] - pq;
+    const float s = lane_seq_sum<false>(P0, y, n, tile, [&](float yq, float pq) {
+      const float t = yq - pq;
       return t * t;
     });
     return (float)sqrt((double)(s / (float)n));
   };
   auto stdv = [&]() {
-    float mean = lane_seq_sum<false>(P0, n, tile, [&](int64_t q, float pq) { return y[q] - pq; });
+    float mean = lane_seq_sum<false>(P0, y, n, tile, [&](float yq, float pq) { return yq - pq; });
     mean = mean / (float)n;
     const float t = (y[n - 1] - p[n - 1]) - mean;
     const float s = (t * t) / (float)(n - 1);
@@ -538,8 +565,7 @@ __device__ float lane_loss_value(int code, const float* __restrict__ y,
   switch (code) {
     case kRmse: return rmse();
     case kRmseNorm: {
-      const float s = lane_seq_sum<true>(P0, n, tile, [&](int64_t q, float pq) {
-        const float tv = y[q];
+      const float s = lane_seq_sum<true>(P0, y, n, tile, [&](float tv, float pq) {
         const float t = (float)((double)(tv - pq) / (1.0 + (double)tv));
         return t * t;
       });
