@@ -93,8 +93,6 @@ __global__ __launch_bounds__(256) void k_split(const float* __restrict__ X, int6
 // wave holds a 64x64 quarter as 2x2 accumulators of v_mfma_f32_32x32x16_bf16
 // (each A and B fragment serves two products).  Fragment of lane l: row (l mod
 // 32) of A, column (l mod 32) of B, k = 8*(l/32) .. +7 -- one bf16x8 of S8.
-// Three waves per SIMD (93 VGPRs, 64 accumulation registers) overlap one
-// wave's fragment loads with another's products.
 // Accumulator element e of lane l is row 8*(e/4) + 4*(l/32) + e%4, column
 // l mod 32.  The tile is stored twice, as D[row][col] and, four rows at a time
 // as float4, as D[col][row]: D is exactly symmetric and the matrix cores do
@@ -119,23 +117,48 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const bf16x8* __restrict__ S8
   const int64_t i0 = bi * 128 + (wave >> 1) * 64;
   const int64_t j0 = bj * 128 + (wave & 1) * 64;
   const size_t pstep = (size_t)nrb * nks * 64;
-  const bf16x8* A[2];
-  const bf16x8* B[2];
+  // The block's 24 fragments of one k-step (A then B; 4 row blocks x 3
+  // planes each; fragment f at lds[buf][f * 64 + lane], 24 KB) are staged in
+  // LDS, double-buffered: every operand is read from L2 once per block
+  // instead of once per wave.  Thread (wave w, lane) loads fragments
+  // f = w + 4i, i < 6 -- one coalesced 1 KB read per wave each -- for the
+  // next k-step while the current one's products run.
+  __shared__ bf16x8 lds[2][24 * 64];
+  const bf16x8* src[6];
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
-    A[u] = S8 + (size_t)(i0 / 32 + u) * nks * 64 + lane;
-    B[u] = S8 + (size_t)(j0 / 32 + u) * nks * 64 + lane;
+  for (int i = 0; i < 6; ++i) {
+    const int f = wave + 4 * i;
+    const int side = f / 12, rbl = (f % 12) / 3, p = f % 3;
+    const int64_t rb = (side ? bj : bi) * 4 + rbl;
+    src[i] = S8 + p * pstep + (size_t)rb * nks * 64 + lane;
   }
+  bf16x8 stage[6];
+  auto fetch = [&](int ks) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) stage[i] = src[i][(size_t)ks * 64];
+  };
+  auto put = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) lds[buf][(wave + 4 * i) * 64 + lane] = stage[i];
+  };
+  const int fa0 = (wave >> 1) * 2 * 3, fb0 = 12 + (wave & 1) * 2 * 3;
   floatx16 acc[2][2] = {};
+  fetch(0);
+  put(0);
+  __syncthreads();
   for (int ks = 0; ks < nks; ++ks) {
-    const size_t ko = (size_t)ks * 64;
+    const int buf = ks & 1;
+    const bool next = ks + 1 < nks;
+    if (next) fetch(ks + 1);
+    // the next step's loads stay ahead of this step's products
+    __builtin_amdgcn_sched_barrier(0);
     bf16x8 fa[2][3], fb[2][3];  // [tile][plane]
 #pragma unroll
     for (int u = 0; u < 2; ++u)
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
-        fa[u][p] = A[u][p * pstep + ko];
-        fb[u][p] = B[u][p * pstep + ko];
+        fa[u][p] = lds[buf][(fa0 + u * 3 + p) * 64 + lane];
+        fb[u][p] = lds[buf][(fb0 + u * 3 + p) * 64 + lane];
       }
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
@@ -152,6 +175,8 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const bf16x8* __restrict__ S8
         c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[0], y[0], c, 0, 0, 0);
         acc[ti][tj] = c;
       }
+    if (next) put(buf ^ 1);  // the other buffer was last read before the previous barrier
+    __syncthreads();
   }
   const bool diag = bi == bj;
   const bool vec = (n & 3) == 0;  // float4 mirror stores stay aligned
