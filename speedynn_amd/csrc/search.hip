@@ -657,7 +657,9 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
   constexpr int SUB = 64 * W;
   constexpr int CH = R * SUB;
   constexpr int LDS_Q = (MULT > 0 ? 2 : 3) * CH;  // queue: index, D, lower bound (backward)
-  constexpr int LDS_E = 4 * K * 64;   // epilogue labels, 4 waves at a time
+  // epilogue labels, 4 waves at a time (only for lists longer than a wave's
+  // pending area, which otherwise holds them)
+  constexpr int LDS_E = K <= 2 * pend_slots<K>() ? 0 : 4 * K * 64;
   __shared__ float lds[LDS_Q > LDS_E ? LDS_Q : LDS_E];
   __shared__ float pend[W * 2 * pend_slots<K>() * 64];  // pending buffers, one area per wave
   __shared__ float Tw[W];
@@ -713,7 +715,9 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
   };
 
   auto ldD = [&](int j) { return j < n ? __builtin_nontemporal_load(Dq + j) : kInf; };
-  auto ldM = [&](int j) { return (MULT < 0 && j < n) ? __builtin_nontemporal_load(Mq + j) : 0.0f; };
+  auto ldM = [&](int j) {
+    return (MULT < 0 && j < n) ? __builtin_nontemporal_load(Mq + j) : 0.0f;
+  };
   const int t0 = threadIdx.x;
   float dn[R], mn[R];  // the next chunk's D (and M), loaded one chunk ahead
 #pragma unroll
@@ -825,11 +829,17 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
     const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
     atomicAdd(a.visited, seen * (unsigned long long)nlive);
   }
-  // epilogue, four waves at a time through the label area (aliases the queue)
-  for (int g = 0; g < W; g += 4) {
-    __syncthreads();
-    if (active && wave >= g && wave < g + 4)
-      finish_query<K>(a, Li, lds + (wave - g) * (K * 64) + lane, q, c, live);
+  // epilogue: the labels go to the wave's own pending area (empty after the
+  // last flush) when it holds K slots, so all waves finish at once; longer
+  // lists take four waves at a time through the label area (aliases the queue)
+  if constexpr (K <= 2 * pend_slots<K>()) {
+    if (active) finish_query<K>(a, Li, buf, q, c, live);
+  } else {
+    for (int g = 0; g < W; g += 4) {
+      __syncthreads();
+      if (active && wave >= g && wave < g + 4)
+        finish_query<K>(a, Li, lds + (wave - g) * (K * 64) + lane, q, c, live);
+    }
   }
 }
 
