@@ -53,6 +53,11 @@ RANDOM = [
     (64, 333, 8, [15], 333, "regression", "backward", 3),
     (65, 130, 70, [7, 3], 130, "classification", "forward", 3),
     (66, 190, 6, [32, 50], 190, "regression", "forward", 3),
+    # longest lists backward (the 64-entry list, and the 50-entry list of k = 33 over two
+    # candidate blocks and two folds), and k = 20 forward over two candidate blocks
+    (67, 260, 20, [64], 260, "classification", "backward", 15),
+    (68, 300, 70, [24, 33], 150, "regression", "backward", 66),
+    (69, 280, 90, [20], 280, "classification", "forward", 4),
 ]
 
 
