@@ -159,3 +159,26 @@ def test_reference_qsort_order_matches_the_c_library(tmp_path):
                    check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0 and out.stdout.startswith("ok"), out.stdout
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says
+    so, before any device call."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_bench_reads_committed_traffic_summaries():
+    """Every workload's bench line carries roofline.traffic from the committed
+    rocprofv3 counter summary (profiles/traffic_search_<workload>.md)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    for w in ("c1", "c2", "c3", "c4"):
+        t = bench.traffic_per_launch(w)
+        assert t is not None and t > 0, w
