@@ -407,11 +407,15 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
                               ctx->errors.as<float>(), ctx->val.as<float>(), ctx->bestk.as<int>(),
                               cand_err, ctx->perm.as<int>(), s));
     HIPCHK(hipEventRecord(u, s));
-    if (fused_seeds)
-      HIPCHK(snn::launch_update_seeds(D, Dlo, ctx->XT.as<float>(), n, st, fwd ? 1 : -1,
-                                      ctx->seeds.as<int>(), pb.npf, s));
-    else
-      HIPCHK(snn::launch_update(D, Dlo, ctx->XT.as<float>(), n, st, -1, fwd ? 1 : -1, s));
+    // the last step's choice is not applied to D: no later search reads it
+    // (the state's pending flag is reset when the next selection starts)
+    if (step + 1 < pb.max_steps) {
+      if (fused_seeds)
+        HIPCHK(snn::launch_update_seeds(D, Dlo, ctx->XT.as<float>(), n, st, fwd ? 1 : -1,
+                                        ctx->seeds.as<int>(), pb.npf, s));
+      else
+        HIPCHK(snn::launch_update(D, Dlo, ctx->XT.as<float>(), n, st, -1, fwd ? 1 : -1, s));
+    }
     for (hipEvent_t e : {a, b, b2, c, u}) marks.push_back(e);
   }
   hipEvent_t e2 = tm.make();

@@ -371,12 +371,11 @@ __global__ __launch_bounds__(256) void k_compact(const float* __restrict__ X,
   if (blockIdx.x == 0 && threadIdx.x == 0) st->pending = 0;
   if (st->stopped) return;
   const int nc = st->ncand;
-  const int64_t total = n * cpad;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t j = t / cpad;
-    const int c = (int)(t - j * cpad);
-    Xc[t] = c < nc ? X[j * d + cand[c]] : 0.0f;
+  // rows across blocks, candidate slots across threads (no index division)
+  for (int64_t j = blockIdx.x; j < n; j += gridDim.x) {
+    const float* __restrict__ xr = X + j * d;
+    float* __restrict__ out = Xc + j * cpad;
+    for (int c = threadIdx.x; c < cpad; c += blockDim.x) out[c] = c < nc ? xr[cand[c]] : 0.0f;
   }
 }
 
@@ -681,9 +680,20 @@ __device__ float loss_value(int code, const float* __restrict__ y, const float* 
                             int64_t n, unsigned int* hist, float* scratch) {
   const int lane = threadIdx.x & 63;
   switch (code) {
-    case kZeroOne: {
+    case kZeroOne: {  // an integer count: any order; eight loads of each row in flight
       int cnt = 0;
-      for (int64_t q = lane; q < n; q += 64) cnt += zero_one_err(y[q], p[q]);
+      int64_t q = lane;
+      for (; q + 7 * 64 < n; q += 8 * 64) {
+        float yv[8], pv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          yv[u] = y[q + u * 64];
+          pv[u] = p[q + u * 64];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) cnt += zero_one_err(yv[u], pv[u]);
+      }
+      for (; q < n; q += 64) cnt += zero_one_err(y[q], p[q]);
       for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
       return (float)cnt / (float)n;
     }
@@ -1040,7 +1050,7 @@ hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
 
 hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* Xc,
                           int64_t n, int d, int cpad, hipStream_t s) {
-  hipLaunchKernelGGL(k_compact, dim3(grid_1d(n * cpad, 256, 16384)), dim3(256), 0, s, X, cand, st,
+  hipLaunchKernelGGL(k_compact, dim3(grid_1d(n, 1, 16384)), dim3(256), 0, s, X, cand, st,
                      Xc, n, d, cpad);
   return hipGetLastError();
 }
