@@ -337,10 +337,12 @@ def bench_gram(args, hip, eng) -> dict:
     hip.free(xp)
     hip.free(dp)
     flops = 2.0 * n * n * d  # algorithmic: one multiply-add per (pair, feature)
-    # products the matrix cores execute: the upper triangle of 128x128 tiles
-    # (diagonal tiles whole), features padded to 16, six split products each
-    nb = (n + 127) // 128
-    executed = 6 * 2.0 * (nb * (nb + 1) // 2) * 128 * 128 * ((d + 15) // 16 * 16)
+    # products the matrix cores execute: the upper triangle of square tiles
+    # (256 points from n = 8192 on, else 128; diagonal tiles whole), features
+    # padded to 16, six split products each
+    tile = 256 if n >= 8192 else 128
+    nb = (n + tile - 1) // tile
+    executed = 6 * 2.0 * (nb * (nb + 1) // 2) * tile * tile * ((d + 15) // 16 * 16)
     t_m, t_e = times["mfma"] / 1e3, times["exact"] / 1e3
     return {"gram": args.gram, "n": n, "d": d,
             "mfma_ms": times["mfma"], "exact_ms": times["exact"],

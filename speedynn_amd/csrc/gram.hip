@@ -89,20 +89,26 @@ __global__ __launch_bounds__(256) void k_split(const float* __restrict__ X, int6
   }
 }
 
-// One 128x128 tile (bi, bj), bi <= bj, of the upper triangle per block; each
-// wave holds a 64x64 quarter as 2x2 accumulators of v_mfma_f32_32x32x16_bf16
+// One (64 TB)-square tile (bi, bj), bi <= bj, of the upper triangle per block
+// of TB x TB waves (TB = 2: 128x128, 4 waves; TB = 4: 256x256, 16 waves); each
+// wave holds a 64x64 part as 2x2 accumulators of v_mfma_f32_32x32x16_bf16
 // (each A and B fragment serves two products).  Fragment of lane l: row (l mod
 // 32) of A, column (l mod 32) of B, k = 8*(l/32) .. +7 -- one bf16x8 of S8.
 // Accumulator element e of lane l is row 8*(e/4) + 4*(l/32) + e%4, column
 // l mod 32.  The tile is stored twice, as D[row][col] and, four rows at a time
 // as float4, as D[col][row]: D is exactly symmetric and the matrix cores do
-// half the work.  A diagonal tile stores only row < col (and 0 on the
-// diagonal).  Blocks map to XCDs in contiguous ranges of the column-major
-// triangle order, so each XCD's L2 serves consecutive tiles of one B panel.
-__global__ __launch_bounds__(256) void k_gram_mfma(const bf16x8* __restrict__ S8,
-                                                   const float* __restrict__ norm, int64_t n,
-                                                   int nrb, int nks, int64_t ntiles,
-                                                   float* __restrict__ D) {
+// half the work.  In a diagonal tile, waves below the diagonal store nothing
+// and waves on it store only row < col (and 0 on the diagonal).  Blocks map to
+// XCDs in contiguous ranges of the column-major triangle order, so each XCD's
+// L2 serves consecutive tiles of one B panel.  The larger tile reads each
+// operand byte for twice as many products.
+template <int TB>
+__global__ __launch_bounds__(64 * TB * TB) void k_gram_mfma(const bf16x8* __restrict__ S8,
+                                                            const float* __restrict__ norm,
+                                                            int64_t n, int nrb, int nks,
+                                                            int64_t ntiles,
+                                                            float* __restrict__ D) {
+  constexpr int TILE = 64 * TB, RB = 2 * TB, NF = 6 * RB, WAVES = TB * TB, PER = NF / WAVES;
   const int64_t per_xcd = (ntiles + 7) / 8;
   const int64_t t = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   if (t >= ntiles) return;
@@ -114,34 +120,35 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const bf16x8* __restrict__ S8
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int64_t i0 = bi * 128 + (wave >> 1) * 64;
-  const int64_t j0 = bj * 128 + (wave & 1) * 64;
+  const int wr = wave / TB, wc = wave % TB;
+  const int64_t i0 = bi * TILE + wr * 64;
+  const int64_t j0 = bj * TILE + wc * 64;
   const size_t pstep = (size_t)nrb * nks * 64;
-  // The block's 24 fragments of one k-step (A then B; 4 row blocks x 3
-  // planes each; fragment f at lds[buf][f * 64 + lane], 24 KB) are staged in
-  // LDS, double-buffered: every operand is read from L2 once per block
-  // instead of once per wave.  Thread (wave w, lane) loads fragments
-  // f = w + 4i, i < 6 -- one coalesced 1 KB read per wave each -- for the
-  // next k-step while the current one's products run.
-  __shared__ bf16x8 lds[2][24 * 64];
-  const bf16x8* src[6];
+  // The block's NF fragments of one k-step (A then B; RB row blocks x 3
+  // planes each; fragment f at lds[buf][f * 64 + lane]) are staged in LDS,
+  // double-buffered: every operand is read from L2 once per block instead of
+  // once per wave.  Thread (wave w, lane) loads fragments f = w + WAVES*i,
+  // i < PER -- one coalesced 1 KB read per wave each -- for the next k-step
+  // while the current one's products run.
+  __shared__ bf16x8 lds[2][NF * 64];
+  const bf16x8* src[PER];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) {
-    const int f = wave + 4 * i;
-    const int side = f / 12, rbl = (f % 12) / 3, p = f % 3;
-    const int64_t rb = (side ? bj : bi) * 4 + rbl;
+  for (int i = 0; i < PER; ++i) {
+    const int f = wave + WAVES * i;
+    const int side = f / (3 * RB), rbl = (f % (3 * RB)) / 3, p = f % 3;
+    const int64_t rb = (side ? bj : bi) * RB + rbl;
     src[i] = S8 + p * pstep + (size_t)rb * nks * 64 + lane;
   }
-  bf16x8 stage[6];
+  bf16x8 stage[PER];
   auto fetch = [&](int ks) {
 #pragma unroll
-    for (int i = 0; i < 6; ++i) stage[i] = src[i][(size_t)ks * 64];
+    for (int i = 0; i < PER; ++i) stage[i] = src[i][(size_t)ks * 64];
   };
   auto put = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < 6; ++i) lds[buf][(wave + 4 * i) * 64 + lane] = stage[i];
+    for (int i = 0; i < PER; ++i) lds[buf][(wave + WAVES * i) * 64 + lane] = stage[i];
   };
-  const int fa0 = (wave >> 1) * 2 * 3, fb0 = 12 + (wave & 1) * 2 * 3;
+  const int fa0 = wr * 2 * 3, fb0 = 3 * RB + wc * 2 * 3;
   floatx16 acc[2][2] = {};
   fetch(0);
   put(0);
@@ -178,7 +185,8 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const bf16x8* __restrict__ S8
     if (next) put(buf ^ 1);  // the other buffer was last read before the previous barrier
     __syncthreads();
   }
-  const bool diag = bi == bj;
+  if (bi == bj && wr > wc) return;  // below the diagonal: the mirror of another wave's part
+  const bool diag = bi == bj && wr == wc;
   const bool vec = (n & 3) == 0;  // float4 mirror stores stay aligned
 #pragma unroll
   for (int tj = 0; tj < 2; ++tj) {
@@ -215,11 +223,16 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const bf16x8* __restrict__ S8
   }
 }
 
-static int gram_blocks_padded(int64_t n) { return (int)((n + 127) / 128); }
+// 256-point tiles from n = 8192 on; smaller problems keep more, smaller tiles.
+static int gram_tile(int64_t n) { return n >= 8192 ? 256 : 128; }
+static int gram_blocks_padded(int64_t n) {
+  const int tile = gram_tile(n);
+  return (int)((n + tile - 1) / tile);
+}
 
 size_t gram_mfma_scratch_bytes(int64_t n, int d) {
   const int nks = (d + 15) / 16;
-  const size_t nrb = (size_t)gram_blocks_padded(n) * 4;
+  const size_t nrb = (size_t)gram_blocks_padded(n) * (gram_tile(n) / 32);
   return 3 * nrb * nks * 64 * sizeof(bf16x8) + ((size_t)n * sizeof(float) + 255) / 256 * 256 +
          (size_t)d * sizeof(double) + 256;
 }
@@ -227,8 +240,9 @@ size_t gram_mfma_scratch_bytes(int64_t n, int d) {
 hipError_t launch_gram_mfma(const float* X, const float* XT, int64_t n, int d, float* D,
                             void* scratch, hipStream_t s) {
   const int nks = (d + 15) / 16;
+  const int tile = gram_tile(n);
   const int nbt = gram_blocks_padded(n);
-  const int nrb = nbt * 4;
+  const int nrb = nbt * (tile / 32);
   char* p = static_cast<char*>(scratch);
   bf16x8* S8 = reinterpret_cast<bf16x8*>(p);
   p += 3 * (size_t)nrb * nks * 64 * sizeof(bf16x8);
@@ -240,8 +254,12 @@ hipError_t launch_gram_mfma(const float* X, const float* XT, int64_t n, int d, f
                      norm);
   const int64_t ntiles = (int64_t)nbt * (nbt + 1) / 2;
   const int64_t grid = (ntiles + 7) / 8 * 8;
-  hipLaunchKernelGGL(k_gram_mfma, dim3((unsigned)grid), dim3(256), 0, s, S8, norm, n, nrb, nks,
-                     ntiles, D);
+  if (tile == 256)
+    hipLaunchKernelGGL(k_gram_mfma<4>, dim3((unsigned)grid), dim3(1024), 0, s, S8, norm, n, nrb,
+                       nks, ntiles, D);
+  else
+    hipLaunchKernelGGL(k_gram_mfma<2>, dim3((unsigned)grid), dim3(256), 0, s, S8, norm, n, nrb,
+                       nks, ntiles, D);
   return hipGetLastError();
 }
 

@@ -209,18 +209,20 @@ def test_compensated_precision_selects_the_same(engine, name):
     assert_same_result(got, expected_arrays(rec), name)
 
 
-@pytest.mark.parametrize("n,d", [(1500, 300), (1001, 37)])
+@pytest.mark.parametrize("n,d", [(1500, 300), (1001, 37), (8300, 37)])
 def test_mfma_gram_within_1e5_of_fp64(engine, n, d):
     """(a) on the matrix cores: the BF16x3 MFMA Gram (SNN_FORM_MFMA) against
     the FP64 oracle, on unit-scale Gaussian features and on features with
     scales 10^U(-2,2).  Tolerance 1e-5 relative; the diagonal is exactly 0 and
-    the matrix exactly symmetric.  Neither n is a multiple of the 128-point
-    tile (partial tiles); n = 1001 is odd, so every mirror store takes the
-    scalar path, and d = 37 pads the last 16-feature step with zeros."""
+    the matrix exactly symmetric.  No n is a multiple of its tile (128 points
+    below n = 8192, 256 from there on: partial tiles); n = 1001 is odd, so
+    every mirror store takes the scalar path, and d = 37 pads the last
+    16-feature step with zeros.  n = 8300 runs the 16-wave 256-point tiles,
+    whose diagonal tiles hold waves below, on and above the diagonal."""
     Z = ds.gaussian_matrix(91, n, d)
     scales = ds.exp2(np.log2(10.0) * (4.0 * ds.uniforms(92, 0, d) - 2.0))
     cases = {"unit": Z, "scaled": (Z.astype(np.float64) * scales[None, :]).astype(np.float32)}
-    rows = [0, 7, 777, n - 1]
+    rows = sorted({0, 7, 70, 200, 777, n // 2 + 3, n - 1})
     for name, X in cases.items():
         print(f"mfma gram {name} n={n} d={d}", flush=True)
         got, lo = engine.distance_rows(X, 0, [], rows, form=2)
