@@ -73,7 +73,18 @@ struct SearchProblem {
   const float* skey;     // [d][n] each feature's values, ascending
   const int* sidx;       // [d][n] their point indices
   const int* cand;       // [d] feature of each candidate slot
+  // Pruned searches: the launch covers queries q0 + i * qstep, i < nq (> 0);
+  // qstep = 0: every point.  The autotuner's trial runs take disjoint parts.
+  int64_t q0, nq, qstep;
 };
+
+// Queries a pruned-search launch covers, and the point of the i-th.
+__host__ __device__ inline int64_t launch_queries(const SearchProblem& a) {
+  return a.qstep ? a.nq : a.n;
+}
+__host__ __device__ inline int64_t query_point(const SearchProblem& a, int64_t i) {
+  return a.qstep ? a.q0 + i * a.qstep : i;
+}
 
 // Neighbour-list length compiled for a largest k value (-1: above 64).
 __host__ __device__ inline int search_list_size(int kmax) {

@@ -124,8 +124,11 @@ double elapsed(hipEvent_t a, hipEvent_t b) {
 // many points, late ones few), not only on the shape.  The first step of a
 // shape (n, d, list size, direction) and step range [2^b, 2^(b+1)) in this
 // process runs every applicable kernel on the same step, timed with HIP
-// events (their results are bit-identical; only the last run counts visits),
-// and the fastest serves the rest of that range.  A kernel that took more
+// events, and the fastest serves the rest of that range.  With nc kernels to
+// try, kernel r serves the queries r, r + nc, r + 2 nc, ... of that step: the
+// parts are disjoint and cover every query once, and every kernel gives the
+// same bits, so the step costs about one run, not nc.  (Below 64 nc points
+// each kernel runs the whole step instead; only the last run counts visits.)  A kernel that took more
 // than twice the winner's time is not tried again on that shape's later
 // ranges (the per-wave kernel, which applies everywhere, always is).
 // SPEEDYNN_TUNING=off keeps the rule by direction.
@@ -198,15 +201,22 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int
     *used |= 1 << (v - 1);
     return SNN_OK;
   }
-  // every applicable kernel on this step; only the last run counts visits
+  // every applicable kernel on its part of this step's queries
   Timing tm;
   hipEvent_t ev[5];
   for (int i = 0; i <= nc; ++i) ev[i] = tm.make();
-  snn::SearchProblem quiet = sl;
-  quiet.visited = nullptr;
+  const bool parts = sl.n >= 64 * (int64_t)nc;
   HIPCHK(hipEventRecord(ev[0], s));
   for (int i = 0; i < nc; ++i) {
-    HIPCHK(launch_variant(i + 1 < nc ? quiet : sl, cands[i], rank, s));
+    snn::SearchProblem run = sl;
+    if (parts) {
+      run.q0 = i;
+      run.qstep = nc;
+      run.nq = (sl.n - i + nc - 1) / nc;
+    } else if (i + 1 < nc) {
+      run.visited = nullptr;
+    }
+    HIPCHK(launch_variant(run, cands[i], rank, s));
     HIPCHK(hipEventRecord(ev[i + 1], s));
     *used |= 1 << (cands[i] - 1);
   }

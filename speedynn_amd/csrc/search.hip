@@ -631,13 +631,15 @@ void k_search_pruned(const SearchProblem a, int cb_major) {
   __shared__ float lds[4 * wave_lds_words<K>()];  // pending buffers, then epilogue labels
   const int wave = uniform(threadIdx.x >> 6);
   const int nblk = a.cpad / 64;
-  const int64_t per_xcd = (a.n + 7) / 8;
+  const int64_t nq = launch_queries(a);
+  const int64_t per_xcd = (nq + 7) / 8;
   const int64_t local = (int64_t)(blockIdx.x / 8) * 4 + wave;
   const int64_t qi = cb_major ? local % per_xcd : local / nblk;
   const int cb = (int)(cb_major ? local / per_xcd : local % nblk);
-  const int64_t q = (int64_t)(blockIdx.x % 8) * per_xcd + qi;
-  if (q >= a.n || cb >= nblk || qi >= per_xcd) return;  // whole wave; no barriers
-  pruned_wave<K, MULT>(a, lds + wave * wave_lds_words<K>() + (threadIdx.x & 63), q, cb);
+  const int64_t i = (int64_t)(blockIdx.x % 8) * per_xcd + qi;
+  if (i >= nq || cb >= nblk || qi >= per_xcd) return;  // whole wave; no barriers
+  pruned_wave<K, MULT>(a, lds + wave * wave_lds_words<K>() + (threadIdx.x & 63),
+                       query_point(a, i), cb);
 }
 
 // Pruned search with the row scan shared by the candidate blocks of a query:
@@ -672,7 +674,7 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
   if (st->stopped) return;
   const int ncand = st->ncand;
   const int n = (int)a.n;  // n < 2^31 (checked on the host)
-  const int q = blockIdx.x;
+  const int q = (int)query_point(a, blockIdx.x);
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   // several workgroups per query split its candidate blocks: blockIdx.y = part,
@@ -860,12 +862,14 @@ __global__ __launch_bounds__(256) void k_search_walk(const SearchProblem a,
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int nblk = a.cpad / 64;
-  const int64_t per_xcd = (a.n + 7) / 8;
+  const int64_t nq = launch_queries(a);
+  const int64_t per_xcd = (nq + 7) / 8;
   const int64_t local = (int64_t)(blockIdx.x / 8) * 4 + wave;
   const int64_t qi = local % per_xcd;
   const int cb = (int)(local / per_xcd);
-  const int64_t q = (int64_t)(blockIdx.x % 8) * per_xcd + qi;
-  if (q >= a.n || cb >= nblk) return;  // whole wave; no barriers
+  const int64_t i = (int64_t)(blockIdx.x % 8) * per_xcd + qi;
+  if (i >= nq || cb >= nblk) return;  // whole wave; no barriers
+  const int64_t q = query_point(a, i);
   float* lab = lds + wave * wave_lds_words<K>() + lane;
   const SelState* st = a.st;
   if (st->stopped) return;
@@ -1132,7 +1136,7 @@ int backward_seed_count(int kmax) {
 template <int K, int W>
 static void launch_pruned_shared(const SearchProblem& a, hipStream_t s) {
   // workgroups per query: enough W-wave groups to cover the candidate blocks
-  const dim3 g((unsigned)a.n, (unsigned)((a.cpad / 64 + W - 1) / W)), b(64 * W);
+  const dim3 g((unsigned)launch_queries(a), (unsigned)((a.cpad / 64 + W - 1) / W)), b(64 * W);
   if (a.mult > 0) hipLaunchKernelGGL((k_search_pruned_shared<K, 1, W>), g, b, 0, s, a);
   else hipLaunchKernelGGL((k_search_pruned_shared<K, -1, W>), g, b, 0, s, a);
 }
@@ -1171,7 +1175,7 @@ static hipError_t launch_pruned_k(const SearchProblem& a, int variant, hipStream
   // 4 independent waves per block; each XCD slice holds per_xcd*nblk waves.
   // forward: gather locality (candidate-block major); backward: the D and M
   // rows dominate (query major)
-  const int64_t per_xcd = (a.n + 7) / 8;
+  const int64_t per_xcd = (launch_queries(a) + 7) / 8;
   const int64_t slice_blocks = (per_xcd * nblk + 3) / 4;
   const dim3 g((unsigned)(8 * slice_blocks)), b(256);
   if (a.mult > 0) hipLaunchKernelGGL((k_search_pruned<K, 1>), g, b, 0, s, a, 1);
@@ -1227,7 +1231,7 @@ bool pruned_shared_applies(const SearchProblem& a) {
 
 template <int K>
 static hipError_t launch_walk_k(const SearchProblem& a, const int* rank, hipStream_t s) {
-  const int64_t per_xcd = (a.n + 7) / 8;
+  const int64_t per_xcd = (launch_queries(a) + 7) / 8;
   const int64_t slice_blocks = (per_xcd * (a.cpad / 64) + 3) / 4;
   hipLaunchKernelGGL(k_search_walk<K>, dim3((unsigned)(8 * slice_blocks)), dim3(256), 0, s, a,
                      rank);
