@@ -908,7 +908,10 @@ __global__ __launch_bounds__(256) void k_select(SelState* __restrict__ st, const
 // Candidate slot order.  The pruned search bounds a block of 64 candidates by
 // the largest bound among them, so features of similar scale are grouped:
 // slots follow the column's sum of squared deviations (float64, fixed
-// reduction order, hence deterministic), ties by index.  key[f] per block.
+// reduction order, hence deterministic) in descending order, ties by index.
+// Descending, the partly filled last block holds the smallest features, and
+// every block's largest member is as small as 64-slot blocks allow.  key[f]
+// per block.
 __global__ __launch_bounds__(256) void k_feature_key(const float* __restrict__ XT, int64_t n,
                                                      double* __restrict__ key) {
   __shared__ double s1[256], s2[256];
@@ -932,17 +935,20 @@ __global__ __launch_bounds__(256) void k_feature_key(const float* __restrict__ X
   if (threadIdx.x == 0) key[blockIdx.x] = s2[0] - s1[0] * s1[0] / (double)n;
 }
 
-// perm[rank(f)] = f, rank by (key, index); identity when key is null.
+// perm[rank(f)] = f, rank by (descending key, index); identity when key is
+// null.  A NaN key (a column holding NaN or infinities) ranks as +inf, so the
+// ranks stay a permutation.
 __global__ __launch_bounds__(256) void k_feature_rank(const double* __restrict__ key, int d,
                                                       int* __restrict__ perm) {
+  const double kInf = __longlong_as_double(0x7ff0000000000000LL);
   for (int f = threadIdx.x; f < d; f += blockDim.x) {
     int r = f;
     if (key) {
       r = 0;
-      const double kf = key[f];
+      const double kf = key[f] != key[f] ? kInf : key[f];
       for (int g = 0; g < d; ++g) {
-        const double kg = key[g];
-        r += (kg < kf || (kg == kf && g < f)) ? 1 : 0;
+        const double kg = key[g] != key[g] ? kInf : key[g];
+        r += (kg > kf || (kg == kf && g < f)) ? 1 : 0;
       }
     }
     perm[r] = f;
