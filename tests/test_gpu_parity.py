@@ -88,6 +88,44 @@ def test_every_candidate_error_matches_oracle(engine, case, form):
                                   o["candidate_errors"][:steps].view(np.uint32))
 
 
+NONFINITE = [
+    # seed, n, d, k_vals, k_fold, model, method, num_features
+    (63, 400, 10, [5, 10, 20], 400, "regression", "forward", 4),
+    (62, 257, 9, [2, 4, 8], 13, "classification", "backward", 4),
+]
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["plain", "precision"])
+@pytest.mark.parametrize("case", NONFINITE, ids=lambda c: f"seed{c[0]}-{c[6]}")
+def test_nonfinite_features_match_oracle(engine, case, form):
+    """X holding NaN in two columns and +-inf in a third: a distance through a
+    NaN or an infinity is NaN or inf, which never passes the strict '<' of the
+    list (cpu.c:149-165), so such points stay out of the lists, and the
+    candidates whose errors come out NaN are never chosen.  The precision
+    form takes its non-finite paths here (no sorted first step, the walk's
+    fallback, NaN slot keys).  Errors are compared bit for bit, NaN with NaN
+    (the payload of a NaN is not part of the result)."""
+    seed, n, d, k_vals, k_fold, model, method, nf = case
+    X, y = _random_case(seed, n, d, model)
+    X = X.copy()
+    X[5, 2] = np.nan
+    X[17, 7] = np.nan
+    X[33, 4], X[34, 4], X[40, 4] = np.inf, -np.inf, np.inf
+    loss = "zero_one" if model == "classification" else "rmse"
+    print(f"nonfinite case seed={seed} n={n} d={d} form={form}", flush=True)
+    o = O.select(X, y, k_vals, nf, k_fold, [loss], False, model, method, record_candidates=True)
+    inp = dict(k_vals=k_vals, loss_functions=[loss], method=method, model_type=model,
+               k_fold=k_fold, num_features=nf, auto_feature=False)
+    g = run(engine, X, y, inp, form, candidate_errors=True)
+    assert_same_result(g, o)
+    steps = g["steps"]
+    got, exp = g["candidate_errors"][:steps], o["candidate_errors"][:steps]
+    assert np.isnan(exp).any()  # the case does reach NaN errors
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(exp))
+    fin = ~np.isnan(exp)
+    np.testing.assert_array_equal(got[fin].view(np.uint32), exp[fin].view(np.uint32))
+
+
 @pytest.mark.parametrize("method", ["forward", "backward"])
 def test_distance_rows_bit_exact(engine, method):
     X, _, _ = ds.regression_sum(70, 500, 40, 3, 0.5)
