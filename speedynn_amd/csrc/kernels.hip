@@ -465,7 +465,8 @@ template <bool REV, class Term>
 __device__ __forceinline__ float lane_seq_sum(const float* __restrict__ P0,
                                               const float* __restrict__ y, int64_t n,
                                               float* __restrict__ tiles, Term term) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // wave-uniform, so the loader / adder split is a scalar branch
+  const int wave = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int64_t ntile = (n + kLossTile - 1) / kLossTile;
   float s = 0.0f;
   // loaders keep tile u in register slot u % kLossAhead: the loads of a tile
@@ -473,13 +474,18 @@ __device__ __forceinline__ float lane_seq_sum(const float* __restrict__ P0,
   // memory latency behind the adder's work on the tiles in between
   float raw[kLossAhead][kLossRows];
   float ys[kLossAhead];  // the label of the lane's point, loaded with its rows
+  // Past the last point a lane loads point n - 1 (stage() writes 0 there):
+  // the loads stay unconditional, so none of them sits in a branch whose end
+  // would wait for every load in flight.
   auto fetch = [&](int64_t t, float (&r_)[kLossRows], float& yv) {  // loader: its rows of tile t
     const int64_t k = t * kLossTile + lane;  // k-th point in summation order
-    const int64_t q = REV ? n - 1 - k : k;
-    const float* base = P0 + (int64_t)(wave * kLossRows) * n + q;
+    const int64_t kc = k < n ? k : n - 1;
+    const int64_t q = REV ? n - 1 - kc : kc;
+    const int lw = wave < kLossLoaders ? wave : 0;  // the adder loads loader 0's rows (unused)
+    const float* base = P0 + (int64_t)(lw * kLossRows) * n + q;
 #pragma unroll
-    for (int r = 0; r < kLossRows; ++r) r_[r] = k < n ? base[(int64_t)r * n] : 0.0f;
-    yv = k < n ? y[q] : 0.0f;
+    for (int r = 0; r < kLossRows; ++r) r_[r] = base[(int64_t)r * n];
+    yv = y[q];
   };
   auto stage = [&](int64_t t, const float (&r_)[kLossRows], float yv, float* buf) {
     const int64_t k = t * kLossTile + lane;
@@ -494,44 +500,69 @@ __device__ __forceinline__ float lane_seq_sum(const float* __restrict__ P0,
     stage(0, raw[0], ys[0], tiles);
   }
   __syncthreads();
-  // one tile: slot U = t % kLossAhead is a compile-time index of raw
-  auto tile_step = [&](int64_t t, auto slot) {
+  // the adder's work on one full tile: all of the row's reads first, then the
+  // dependent adds, so the LDS latency is paid once per tile instead of once
+  // per four terms
+  auto add_full = [&](const float* row) {
+    float4 v[kLossTile / 4];
+#pragma unroll
+    for (int i = 0; i < kLossTile / 4; ++i) v[i] = *reinterpret_cast<const float4*>(row + 4 * i);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < kLossTile / 4; ++i) {
+      s = s + v[i].x;
+      s = s + v[i].y;
+      s = s + v[i].z;
+      s = s + v[i].w;
+    }
+  };
+  // one tile: slot U = t % kLossAhead is a compile-time index of raw.  FULL:
+  // tiles t .. t + kLossAhead exist and t is whole, so the step has no
+  // branch but the loader / adder split (a branch around the loads would make
+  // the compiler wait for every load in flight, or copy the slots)
+  auto tile_step = [&](int64_t t, auto slot, auto full) {
     constexpr int U = decltype(slot)::value;
-    if (t >= ntile) return;  // the same for every wave of the block
+    constexpr bool FULL = decltype(full)::value;
+    if (!FULL && t >= ntile) return;  // the same for every wave of the block
     const float* cur = tiles + (t & 1) * 64 * kLossStride;
     if (wave == kLossLoaders) {
       const float* row = cur + lane * kLossStride;
-      if (n - t * kLossTile >= kLossTile) {
-        // all of the row's reads first, then the dependent adds: the LDS
-        // latency is paid once per tile instead of once per four terms
-        float4 v[kLossTile / 4];
-#pragma unroll
-        for (int i = 0; i < kLossTile / 4; ++i) v[i] = *reinterpret_cast<const float4*>(row + 4 * i);
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int i = 0; i < kLossTile / 4; ++i) {
-          s = s + v[i].x;
-          s = s + v[i].y;
-          s = s + v[i].z;
-          s = s + v[i].w;
-        }
+      if (FULL || n - t * kLossTile >= kLossTile) {
+        add_full(row);
       } else {
         const int cnt = (int)(n - t * kLossTile);
         for (int i = 0; i < cnt; ++i) s = s + row[i];
       }
-    } else if (t + 1 < ntile) {
+    } else if (FULL || t + 1 < ntile) {
       stage(t + 1, raw[(U + 1) % kLossAhead], ys[(U + 1) % kLossAhead],
             tiles + ((t + 1) & 1) * 64 * kLossStride);
-      if (t + kLossAhead < ntile) fetch(t + kLossAhead, raw[U], ys[U]);  // tile t's slot is free
+    }
+    if (FULL) {
+      // tile t's slot is free.  Every wave loads (the adder's loads are never
+      // used), so the slot registers are written outside the loader branch
+      // and the compiler neither copies them nor waits on them at its end.
+      fetch(t + kLossAhead, raw[U], ys[U]);
+    } else if (wave != kLossLoaders && t + 1 < ntile) {
+      // past the last tile the loads repeat it (never staged)
+      fetch(t + kLossAhead < ntile ? t + kLossAhead : ntile - 1, raw[U], ys[U]);
     }
     __syncthreads();  // tile t consumed, tile t + 1 staged
   };
   static_assert(kLossAhead == 4, "tile_step is unrolled for four slots");
-  for (int64_t t0 = 0; t0 < ntile; t0 += kLossAhead) {
-    tile_step(t0, std::integral_constant<int, 0>{});
-    tile_step(t0 + 1, std::integral_constant<int, 1>{});
-    tile_step(t0 + 2, std::integral_constant<int, 2>{});
-    tile_step(t0 + 3, std::integral_constant<int, 3>{});
+  using Yes = std::integral_constant<bool, true>;
+  using No = std::integral_constant<bool, false>;
+  int64_t t0 = 0;
+  for (; t0 + 2 * kLossAhead <= ntile; t0 += kLossAhead) {
+    tile_step(t0, std::integral_constant<int, 0>{}, Yes{});
+    tile_step(t0 + 1, std::integral_constant<int, 1>{}, Yes{});
+    tile_step(t0 + 2, std::integral_constant<int, 2>{}, Yes{});
+    tile_step(t0 + 3, std::integral_constant<int, 3>{}, Yes{});
+  }
+  for (; t0 < ntile; t0 += kLossAhead) {
+    tile_step(t0, std::integral_constant<int, 0>{}, No{});
+    tile_step(t0 + 1, std::integral_constant<int, 1>{}, No{});
+    tile_step(t0 + 2, std::integral_constant<int, 2>{}, No{});
+    tile_step(t0 + 3, std::integral_constant<int, 3>{}, No{});
   }
   return s;
 }
