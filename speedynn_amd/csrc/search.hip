@@ -467,33 +467,27 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
 
   auto flush = [&]() { T = flush_pending<K, MULT>(buf, pc, Ld, Li, lim, thr, T, live); };
 
-  // D (and M) of 64 points per chunk; a window of 4 chunks is in flight.  The
-  // rows stream through once (non-temporal), leaving the caches to Xc.
-  auto ldD = [&](int64_t j) { return j < n ? __builtin_nontemporal_load(Dq + j) : kInf; };
-  auto ldM = [&](int64_t j) {
-    return (MULT < 0 && j < n) ? __builtin_nontemporal_load(Mq + j) : 0.0f;
+  // D (and M) of 64 points per chunk, four chunks in flight in slots that
+  // stay in place: the scan loop is unrolled by four, because rotating the
+  // window through register copies would wait for the load just issued.  A
+  // lane past the last point loads point n - 1 (its ballot bit is off): a
+  // load that a branch may skip leaves the compiler unable to count the
+  // loads in flight, and it waits for all of them.  The rows stream through
+  // once (non-temporal), leaving the caches to Xc.
+  const int n32 = (int)n;  // n < 2^31 (checked on the host)
+  auto ldD = [&](int j) { return __builtin_nontemporal_load(Dq + (j < n32 ? j : n32 - 1)); };
+  auto ldM = [&](int j) {
+    return MULT < 0 ? __builtin_nontemporal_load(Mq + (j < n32 ? j : n32 - 1)) : 0.0f;
   };
-  float d0 = ldD(lane), d1 = ldD(64 + lane), d2 = ldD(128 + lane), d3 = ldD(192 + lane);
-  float m0 = ldM(lane), m1 = ldM(64 + lane), m2 = ldM(128 + lane), m3 = ldM(192 + lane);
+  float dsl[4], msl[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    dsl[u] = ldD(64 * u + lane);
+    msl[u] = ldM(64 * u + lane);
+  }
   const uint32_t ucpad = (uint32_t)cpad;  // n * cpad < 2^31 (checked on the host)
-  for (int64_t j0 = 0; j0 < n; j0 += 64) {
-    const int64_t jl = j0 + lane;
-    const float dcur = d0;
-    const float lb = MULT > 0 ? d0 : d0 - m0;
-    d0 = d1;
-    d1 = d2;
-    d2 = d3;
-    d3 = ldD(jl + 256);
-    if (MULT < 0) {
-      m0 = m1;
-      m1 = m2;
-      m2 = m3;
-      m3 = ldM(jl + 256);
-    }
-    const bool ok = jl < n && (jl < flo || jl >= fhi) && lb <= T;
-    unsigned long long mask = __ballot(ok);
-    if (!mask) continue;
-    seen += __popcll(mask);
+  // the visits of one chunk: the points of `mask`, ascending
+  auto visit = [&](int j0, float dcur, unsigned long long mask) {
     while (mask) {
       constexpr int BATCH = 8;
       int js[BATCH];
@@ -505,7 +499,7 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
         if (mask) {
           const int b = __ffsll((long long)mask) - 1;
           mask &= mask - 1;
-          js[u] = (int)j0 + b;
+          js[u] = j0 + b;
           dqs[u] = lane_value(dcur, b);
           ts[u] = Xl[(uint32_t)js[u] * ucpad];
           cnt = u + 1;
@@ -534,6 +528,24 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
         }
         if (__ballot(pc > pend_slots<K>() - BATCH)) flush();
       }
+    }
+  };
+  const int flo32 = (int)flo, fhi32 = (int)fhi;
+  for (int g0 = 0; g0 < n32; g0 += 256) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int jl = g0 + 64 * u + lane;
+      const float lb = MULT > 0 ? dsl[u] : dsl[u] - msl[u];
+      const bool ok = jl < n32 && (jl < flo32 || jl >= fhi32) && lb <= T;
+      const unsigned long long mask = __ballot(ok);
+      if (mask) {
+        seen += __popcll(mask);
+        visit(g0 + 64 * u, dsl[u], mask);
+      }
+      // the slot's next chunk, issued once its value is dead, so the load
+      // lands in the slot's own register
+      dsl[u] = ldD(jl + 256);
+      if (MULT < 0) msl[u] = ldM(jl + 256);
     }
   }
   flush();
@@ -716,9 +728,10 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
     if (lane == 0) Tw[wave] = T;
   };
 
-  auto ldD = [&](int j) { return j < n ? __builtin_nontemporal_load(Dq + j) : kInf; };
+  const int n32 = (int)n;  // n < 2^31 (checked on the host)
+  auto ldD = [&](int j) { return j < n32 ? __builtin_nontemporal_load(Dq + j) : kInf; };
   auto ldM = [&](int j) {
-    return (MULT < 0 && j < n) ? __builtin_nontemporal_load(Mq + j) : 0.0f;
+    return (MULT < 0 && j < n32) ? __builtin_nontemporal_load(Mq + j) : 0.0f;
   };
   const int t0 = threadIdx.x;
   float dn[R], mn[R];  // the next chunk's D (and M), loaded one chunk ahead
