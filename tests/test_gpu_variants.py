@@ -65,3 +65,28 @@ def test_forced_pruned_kernel_matches_oracle(engine, monkeypatch, case, kernel):
     }[kernel]
     if applies:
         assert used & want, (kernel, used)
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"seed{c[0]}")
+def test_autotuned_selection_matches_oracle(engine, monkeypatch, case):
+    """The autotuner on (no kernel forced): the first step of each step range
+    runs every applicable kernel, kernel r on the queries r, r + nc, ... of
+    that step, so that step's results come from several kernels at once
+    (up to four here: per-wave, shared, split and the walk).  Every step,
+    tuning or not, must give the oracle's errors bit for bit.  These shapes
+    appear in no other autotuned test, so the tuning steps run here."""
+    seed, n, d, k, method, model, nf = case
+    X, y = _data(seed, n, d, model)
+    loss = "zero_one" if model == "classification" else "rmse"
+    print(f"autotuned seed={seed} n={n} d={d} {method}", flush=True)
+    o = O.select(X, y, [k], nf, n, [loss], False, model, method, record_candidates=True)
+    monkeypatch.delenv("SPEEDYNN_PRUNED", raising=False)
+    monkeypatch.delenv("SPEEDYNN_TUNING", raising=False)
+    g = engine.select(X, y, [k], [LOSS[loss]], method=1 if method == "forward" else 0,
+                      model_type=1 if model == "classification" else 0, k_fold=n,
+                      num_features=nf, form=1, candidate_errors=True)
+    assert_same_result(g, o)
+    steps = g["steps"]
+    np.testing.assert_array_equal(g["candidate_errors"][:steps].view(np.uint32),
+                                  o["candidate_errors"][:steps].view(np.uint32))
+    assert bin(g["pruned_kernel"]).count("1") >= 2, g["pruned_kernel"]  # tuning ran
