@@ -239,3 +239,49 @@ def test_oracle_matches_reference_random(seed):
         args = (X, y, [1, 2, 4, 5], 4, 9, ["zero_one", "rmse"], seed == 53,
                 "classification", method)
         assert_same_result(O.select(*args), O.ref_select(*args), f"{seed} {method}")
+
+
+# ------------------------------------------------- FP64 distance rows
+
+def _rel_err(got, ref):
+    nz = ref > 0
+    return float((np.abs(got[nz] - ref[nz]) / ref[nz]).max())
+
+
+def _f64_check(row, X, q, feats):
+    """The float64 reference for a row: numpy's sum over the subset's squared
+    float64 differences (util.c:10-17 evaluated in exact-enough arithmetic)."""
+    Xs = X[:, feats].astype(np.float64)
+    ref = ((Xs[q] - Xs) ** 2).sum(axis=1)
+    return _rel_err(np.asarray(row, np.float64), ref) < 1e-13
+
+
+@pytest.mark.parametrize("feats", [[7, 3, 33, 0, 19], list(range(40)), [39, 38, 2]],
+                         ids=["forward-subset", "all-features", "backward-tail"])
+def test_distance_row_f64_matches_numpy_float64(feats):
+    """oracle_distance_row_f64 is the FP64 yardstick of the MFMA Gram and of
+    the compensated update: it must agree with a float64 numpy computation
+    to round-off, on features of scales 10^-2..10^2."""
+    Z = ds.gaussian_matrix(93, 300, 40)
+    scales = ds.exp2(np.log2(10.0) * (4.0 * ds.uniforms(94, 0, 40) - 2.0))
+    X = (Z.astype(np.float64) * scales[None, :]).astype(np.float32)
+    for q in (0, 151, 299):
+        row = O.distance_row_f64(X, q, feats)
+        assert row[q] == 0.0
+        assert _f64_check(row, X, q, feats)
+
+
+def test_distance_row_f64_planted_mistakes_fail():
+    """Plausible slips in an FP64 reference -- float32 accumulation, a dropped
+    feature, a transposed operand (row of another query) -- fail the check."""
+    Z = ds.gaussian_matrix(95, 200, 30)
+    scales = ds.exp2(np.log2(10.0) * (4.0 * ds.uniforms(96, 0, 30) - 2.0))
+    X = (Z.astype(np.float64) * scales[None, :]).astype(np.float32)
+    feats, q = list(range(30)), 17
+    f32 = np.zeros(200, np.float32)
+    for f in feats:
+        t = (X[q, f] - X[:, f]).astype(np.float32)
+        f32 = (f32 + t * t).astype(np.float32)
+    assert not _f64_check(f32.astype(np.float64), X, q, feats)
+    assert not _f64_check(O.distance_row_f64(X, q, feats[:-1]), X, q, feats)
+    assert not _f64_check(O.distance_row_f64(X, q + 1, feats), X, q, feats)
