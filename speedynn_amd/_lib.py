@@ -12,7 +12,8 @@ import threading
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libspeedynn_amd.so")
+# SPEEDYNN_LIB names another build of the same library (A/B measurements).
+LIB_PATH = os.environ.get("SPEEDYNN_LIB") or os.path.join(HERE, "libspeedynn_amd.so")
 
 SNN_OK, SNN_EINVAL, SNN_ENOMEM, SNN_EDEVICE, SNN_ESTATE, SNN_ENOTSUP = 0, -1, -2, -3, -4, -5
 FORMS = {"plain": 0, "precision": 1}

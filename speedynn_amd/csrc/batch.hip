@@ -374,7 +374,7 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     float* Dlo = sl.comp ? at<float>(cache.arena, sl.Dlo) : nullptr;
     HIPCHK(snn::launch_transpose(sl.dX, XT, pb.n, pb.d, s));
     HIPCHK(snn::launch_feature_order(XT, pb.n, pb.d, at<double>(cache.arena, sl.key),
-                                     at<int>(cache.arena, sl.perm), pruned, s));
+                                     at<int>(cache.arena, sl.perm), nullptr, pruned, s));
     HIPCHK(snn::launch_init_state(at<SelState>(cache.arena, sl.state), pb,
                                   at<int>(cache.arena, sl.cand), at<int>(cache.arena, sl.sel),
                                   at<int>(cache.arena, sl.ordering),

@@ -48,6 +48,18 @@ struct Problem {
   int num_desired;
 };
 
+// Per-row lists of the points a forward pruned search may visit, written by
+// the distance update that precedes the search: for row q the entries
+// e < count[q] of idx/val[q * cap + e] are the eligible points j (outside q's
+// fold), ascending, with D_qj <= the row's largest warm bound; count[q] = -1:
+// no list (no finite warm bound, or more than cap points), scan the row.
+struct RowList {
+  int* idx;
+  float* val;
+  int* count;
+  int64_t cap;
+};
+
 // One selection problem as the search kernels see it (passed by value to a
 // single-problem launch, or as an array to the persistent batched launch).
 struct SearchProblem {
@@ -76,6 +88,17 @@ struct SearchProblem {
   // Pruned searches: the launch covers queries q0 + i * qstep, i < nq (> 0);
   // qstep = 0: every point.  The autotuner's trial runs take disjoint parts.
   int64_t q0, nq, qstep;
+  // Warm bounds (pruned searches; null tauw: none).  tauw[q][r] is the K-th
+  // distance of the feature of rank r (iperm) for query q at the last step
+  // that searched it (kMaxF: none); forward bounds add the largest square the
+  // feature chosen since can contribute, from frange = {min, max} of its
+  // column in XT.
+  float* tauw;
+  const int* iperm;
+  const float* XT;
+  const float* frange;
+  int d;
+  RowList rows;  // forward: short lists instead of D rows where count >= 0 (idx null: none)
 };
 
 // Queries a pruned-search launch covers, and the point of the i-th.

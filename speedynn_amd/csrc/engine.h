@@ -65,11 +65,25 @@ struct BatchCache {
 
 }  // namespace snn
 
+// Phase-timing events of one context, created once and reused by every
+// call: a ring of kRingSteps steps x 5 marks (a step's slots are reused
+// kRingSteps steps later, after its times have been read), the tuner's
+// trial marks and the call's start/matrix/end marks.
+struct EventPool {
+  static constexpr int kRingSteps = 64;
+  static constexpr int kMarks = 5;
+  std::vector<hipEvent_t> ring;  // kRingSteps * kMarks
+  hipEvent_t tune[5] = {};
+  hipEvent_t span[3] = {};
+  bool ready = false;
+};
+
 struct snn_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   snn::DevBuf X, y, XT, D, Dlo, Xc, P, err, val, bestk, cand, sel, ordering, errors, cand_err,
       kvals, losses, state, scratch, seeds, visited, perm, key, M, med, skey, sidx, sort_tmp,
-      rank;
+      rank, tauw, iperm, frange, lidx, lval, lcount;
   snn::BatchCache batch;
+  EventPool events;
 };

@@ -5,6 +5,7 @@
 // steps without synchronising and reads the results once at the end.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -117,6 +118,29 @@ double elapsed(hipEvent_t a, hipEvent_t b) {
   return 0.0;
 }
 
+// The context's event pool, created on first use (hipSetDevice done).
+int ensure_events(snn_ctx* ctx) {
+  EventPool& ep = ctx->events;
+  if (ep.ready) return SNN_OK;
+  ep.ring.assign((size_t)EventPool::kRingSteps * EventPool::kMarks, nullptr);
+  for (hipEvent_t& e : ep.ring) HIPCHK(hipEventCreate(&e));
+  for (hipEvent_t& e : ep.tune) HIPCHK(hipEventCreate(&e));
+  for (hipEvent_t& e : ep.span) HIPCHK(hipEventCreate(&e));
+  ep.ready = true;
+  return SNN_OK;
+}
+
+void destroy_events(snn_ctx* ctx) {
+  EventPool& ep = ctx->events;
+  for (hipEvent_t e : ep.ring)
+    if (e) hipEventDestroy(e);
+  for (hipEvent_t e : ep.tune)
+    if (e) hipEventDestroy(e);
+  for (hipEvent_t e : ep.span)
+    if (e) hipEventDestroy(e);
+  ep = EventPool{};
+}
+
 // Autotuner of the pruned search.  Which kernel is faster -- a wave per
 // (query, candidate block), a workgroup per query sharing the row scan, or
 // that scan split over two workgroups per query -- depends on the data (how
@@ -131,7 +155,8 @@ double elapsed(hipEvent_t a, hipEvent_t b) {
 // each kernel runs the whole step instead; only the last run counts visits.)  A kernel that took more
 // than twice the winner's time is not tried again on that shape's later
 // ranges (the per-wave kernel, which applies everywhere, always is).
-// SPEEDYNN_TUNING=off keeps the rule by direction.
+// SPEEDYNN_TUNING=off keeps the rule by direction; so does a call without
+// events (`tune_ev` null), which runs a kernel already chosen for the key.
 std::mutex g_tune_mu;
 std::map<std::tuple<int64_t, int, int, int, int>, int> g_tuned;
 std::map<std::tuple<int64_t, int, int, int>, int> g_slow;  // bit v: variant v lost by 2x
@@ -145,7 +170,7 @@ hipError_t launch_variant(const snn::SearchProblem& sl, int variant, const int* 
 }
 
 int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int* rank,
-                        hipStream_t s, int* used) {
+                        hipStream_t s, int* used, hipEvent_t* tune_ev) {
   static const bool off = [] {
     const char* e = std::getenv("SPEEDYNN_TUNING");
     return e && !std::strcmp(e, "off");
@@ -180,7 +205,7 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int
     auto it = g_tuned.find(key);
     if (it != g_tuned.end()) {
       variant = it->second;
-    } else {
+    } else if (tune_ev) {
       auto sw = g_slow.find(shape);
       const int slow = sw == g_slow.end() ? 0 : sw->second;
       int kept = 0;
@@ -202,9 +227,7 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int
     return SNN_OK;
   }
   // every applicable kernel on its part of this step's queries
-  Timing tm;
-  hipEvent_t ev[5];
-  for (int i = 0; i <= nc; ++i) ev[i] = tm.make();
+  hipEvent_t* ev = tune_ev;
   const bool parts = sl.n >= 64 * (int64_t)nc;
   HIPCHK(hipEventRecord(ev[0], s));
   for (int i = 0; i < nc; ++i) {
@@ -241,10 +264,106 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int
   return SNN_OK;
 }
 
+// Device buffers of one selection.
+struct Work {
+  float *XT = nullptr, *D = nullptr, *Dlo = nullptr, *Xc = nullptr, *P = nullptr;
+  float *err = nullptr, *val = nullptr, *errors = nullptr, *cand_err = nullptr;
+  int *bestk = nullptr, *cand = nullptr, *sel = nullptr, *ordering = nullptr;
+  int *kvals = nullptr, *losses = nullptr, *perm = nullptr, *iperm = nullptr;
+  SelState* st = nullptr;
+  double* key = nullptr;
+  float *M = nullptr, *skey = nullptr, *med = nullptr, *tauw = nullptr, *frange = nullptr;
+  int *lidx = nullptr, *lcount = nullptr;
+  float* lval = nullptr;
+  int *seeds = nullptr, *sidx = nullptr, *rank = nullptr;
+  unsigned long long* visited = nullptr;
+  void* sort_tmp = nullptr;
+};
+
+// Binds each requested buffer either to the context's grow-only device
+// buffer of that role (arena null: hipMalloc when it must grow) or to a
+// 256-byte aligned range of a caller-provided arena of `cap` bytes (no
+// allocation; the request fails when the ranges do not fit).
+class Binder {
+ public:
+  Binder(char* arena, size_t cap) : arena_(arena), cap_(cap) {}
+  template <class T>
+  void add(snn::DevBuf& buf, size_t bytes, T** slot) {
+    items_.push_back({&buf, bytes ? bytes : 256, reinterpret_cast<void**>(slot)});
+  }
+  size_t total() const {
+    size_t off = 0;
+    for (const Item& it : items_) off = (off + 255) / 256 * 256 + it.bytes;
+    return off;
+  }
+  int bind() {
+    if (!arena_) {
+      for (const Item& it : items_) {
+        HIPCHK(it.buf->ensure(it.bytes));
+        *it.slot = it.buf->p;
+      }
+      return SNN_OK;
+    }
+    const size_t need = total();
+    if (need > cap_)
+      return fail(SNN_EINVAL, "problem needs " + std::to_string(need) +
+                                  " bytes of device work memory; the fixed scratch holds " +
+                                  std::to_string(cap_));
+    size_t off = 0;
+    for (const Item& it : items_) {
+      off = (off + 255) / 256 * 256;
+      *it.slot = arena_ + off;
+      off += it.bytes;
+    }
+    return SNN_OK;
+  }
+
+ private:
+  struct Item {
+    snn::DevBuf* buf;
+    size_t bytes;
+    void** slot;
+  };
+  char* arena_;
+  size_t cap_;
+  std::vector<Item> items_;
+};
+
+// Phase times of the steps issued so far, read from the event ring as the
+// steps' slots come up for reuse (and at the end).
+struct PhaseTimes {
+  double search = 0, update = 0, tally = 0, bound = 0, search_first = 0;
+};
+
+int retire_step(snn_ctx* ctx, int t, int issued, hipEvent_t end, PhaseTimes* pt, bool trace) {
+  constexpr int R = EventPool::kRingSteps, M = EventPool::kMarks;
+  const hipEvent_t* m = &ctx->events.ring[(size_t)(t % R) * M];
+  const hipEvent_t next = t + 1 < issued ? ctx->events.ring[(size_t)((t + 1) % R) * M] : end;
+  HIPCHK(hipEventSynchronize(next));
+  const double tally = elapsed(m[0], m[1]) + elapsed(m[3], m[4]);
+  const double bound = elapsed(m[1], m[2]);
+  const double search = elapsed(m[2], m[3]);
+  const double update = elapsed(m[4], next);
+  pt->tally += tally;
+  pt->bound += bound;
+  pt->search += search;
+  pt->update += update;
+  if (t == 0) pt->search_first = search;
+  if (trace)
+    std::fprintf(stderr, "speedynn step %d: search %.3f ms, bound %.3f, update %.3f, tally %.3f\n",
+                 t, search, bound, update, tally);
+  return SNN_OK;
+}
+
+// One selection.  arena == null: device buffers from the context (allocated
+// on growth), phase timing and autotuning on.  arena != null: every device
+// buffer is a range of the arena (cap bytes), no allocation, no events (the
+// summary's times stay 0) and no tuning trials.
 int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy,
                 const int* k_vals, int num_k, const int* loss_functions, int num_loss,
                 const snn_params* prm, int* selected_dimensions, int* selected_ordering,
-                float* selected_errors, float* candidate_errors, snn_summary* summary) {
+                float* selected_errors, float* candidate_errors, snn_summary* summary,
+                char* arena = nullptr, size_t cap = 0) {
   if (!ctx) return fail(SNN_EINVAL, "null context");
   Problem pb{};
   int rc = snn::validate_problem(n, d, k_vals, num_k, loss_functions, num_loss, prm, &pb);
@@ -256,96 +375,114 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   // selection reads it)
   const bool pruned = prm->form == SNN_FORM_PRECISION;
   const bool comp = pruned && prm->compensated;
+  const bool timed = arena == nullptr;
 
   HIPCHK(hipSetDevice(ctx->device));
   const size_t nn = (size_t)n * (size_t)n;
-  HIPCHK(ctx->XT.ensure(sizeof(float) * (size_t)n * d));
-  HIPCHK(ctx->D.ensure(sizeof(float) * nn));
-  if (comp) HIPCHK(ctx->Dlo.ensure(sizeof(float) * nn));
-  HIPCHK(ctx->Xc.ensure(sizeof(float) * (size_t)n * pb.cpad));
-  HIPCHK(ctx->P.ensure(sizeof(float) * (size_t)num_k * pb.cpad * n));
-  HIPCHK(ctx->err.ensure(sizeof(float) * (size_t)num_loss * num_k * pb.cpad));
-  HIPCHK(ctx->val.ensure(sizeof(float) * (size_t)num_loss * pb.cpad));
-  HIPCHK(ctx->bestk.ensure(sizeof(int) * pb.cpad));
-  HIPCHK(ctx->cand.ensure(sizeof(int) * d));
-  HIPCHK(ctx->sel.ensure(sizeof(int) * d));
-  HIPCHK(ctx->ordering.ensure(sizeof(int) * d));
-  HIPCHK(ctx->errors.ensure(sizeof(float) * (size_t)num_loss * d));
-  HIPCHK(ctx->kvals.ensure(sizeof(int) * num_k));
-  HIPCHK(ctx->losses.ensure(sizeof(int) * num_loss));
-  HIPCHK(ctx->state.ensure(sizeof(SelState)));
-  HIPCHK(ctx->perm.ensure(sizeof(int) * d));
-  HIPCHK(ctx->key.ensure(sizeof(double) * d));
-  // exact pruning with bounds from 128 seeds per query (backward also keeps M,
-  // the max single-feature term, for its lower bound)
-  if (pruned && !fwd) HIPCHK(ctx->M.ensure(sizeof(float) * nn));
-  if (pruned) {
-    HIPCHK(ctx->seeds.ensure(sizeof(int) * (size_t)n * 128));
-    HIPCHK(ctx->visited.ensure(sizeof(unsigned long long)));
-    HIPCHK(snn::launch_fill_u64(ctx->visited.as<unsigned long long>(), 1, ctx->stream));
-  }
   // precision form, forward: the first step runs over sorted feature columns
   const bool sorted_first = pruned && fwd;
   size_t sort_bytes = 0;
   if (sorted_first) {
-    HIPCHK(ctx->skey.ensure(sizeof(float) * (size_t)n * d));
-    HIPCHK(ctx->sidx.ensure(sizeof(int) * (size_t)n * d));
     sort_bytes = snn::sort_columns_temp_bytes(n, d);
     if (!sort_bytes) return fail(SNN_EDEVICE, "segmented sort: no temporary storage size");
-    HIPCHK(ctx->sort_tmp.ensure(sort_bytes));
-    HIPCHK(ctx->rank.ensure(sizeof(int) * (size_t)n));
   }
-  float* med = nullptr;  // median losses: sort space for NaN-holding error vectors
-  if (snn::needs_median_scratch(loss_functions, num_loss)) {
-    HIPCHK(ctx->med.ensure(sizeof(float) * 2 * (size_t)n * pb.cpad * num_k));
-    med = ctx->med.as<float>();
+  const bool median = snn::needs_median_scratch(loss_functions, num_loss);
+  Work w;
+  Binder bd(arena, cap);
+  bd.add(ctx->XT, sizeof(float) * (size_t)n * d, &w.XT);
+  bd.add(ctx->D, sizeof(float) * nn, &w.D);
+  if (comp) bd.add(ctx->Dlo, sizeof(float) * nn, &w.Dlo);
+  bd.add(ctx->Xc, sizeof(float) * (size_t)n * pb.cpad, &w.Xc);
+  bd.add(ctx->P, sizeof(float) * (size_t)num_k * pb.cpad * n, &w.P);
+  bd.add(ctx->err, sizeof(float) * (size_t)num_loss * num_k * pb.cpad, &w.err);
+  bd.add(ctx->val, sizeof(float) * (size_t)num_loss * pb.cpad, &w.val);
+  bd.add(ctx->bestk, sizeof(int) * pb.cpad, &w.bestk);
+  bd.add(ctx->cand, sizeof(int) * d, &w.cand);
+  bd.add(ctx->sel, sizeof(int) * d, &w.sel);
+  bd.add(ctx->ordering, sizeof(int) * d, &w.ordering);
+  bd.add(ctx->errors, sizeof(float) * (size_t)num_loss * d, &w.errors);
+  bd.add(ctx->kvals, sizeof(int) * num_k, &w.kvals);
+  bd.add(ctx->losses, sizeof(int) * num_loss, &w.losses);
+  bd.add(ctx->state, sizeof(SelState), &w.st);
+  bd.add(ctx->perm, sizeof(int) * d, &w.perm);
+  bd.add(ctx->key, sizeof(double) * d, &w.key);
+  // exact pruning with bounds from 128 seeds per query (backward also keeps M,
+  // the max single-feature term, for its lower bound) and warm bounds from
+  // the previous step's K-th distances
+  if (pruned && !fwd) bd.add(ctx->M, sizeof(float) * nn, &w.M);
+  if (pruned) {
+    bd.add(ctx->seeds, sizeof(int) * (size_t)n * 128, &w.seeds);
+    bd.add(ctx->visited, sizeof(unsigned long long), &w.visited);
+    bd.add(ctx->tauw, sizeof(float) * (size_t)n * d, &w.tauw);
+    bd.add(ctx->iperm, sizeof(int) * d, &w.iperm);
+    bd.add(ctx->frange, sizeof(float) * 2, &w.frange);
   }
-  float* cand_err = nullptr;
-  if (candidate_errors) {
-    HIPCHK(ctx->cand_err.ensure(sizeof(float) * (size_t)d * d));
-    cand_err = ctx->cand_err.as<float>();
+  // forward precision form: the update writes each row's list of points the
+  // next search may visit (up to n/8 entries, at least 256, per row)
+  const int64_t rows_cap =
+      (pruned && fwd && n % 4 == 0) ? std::min<int64_t>(n, std::max<int64_t>(256, n / 8)) : 0;
+  if (rows_cap) {
+    bd.add(ctx->lidx, sizeof(int) * (size_t)n * rows_cap, &w.lidx);
+    bd.add(ctx->lval, sizeof(float) * (size_t)n * rows_cap, &w.lval);
+    bd.add(ctx->lcount, sizeof(int) * (size_t)n, &w.lcount);
+  }
+  if (sorted_first) {
+    bd.add(ctx->skey, sizeof(float) * (size_t)n * d, &w.skey);
+    bd.add(ctx->sidx, sizeof(int) * (size_t)n * d, &w.sidx);
+    bd.add(ctx->sort_tmp, sort_bytes, &w.sort_tmp);
+    bd.add(ctx->rank, sizeof(int) * (size_t)n, &w.rank);
+  }
+  // median losses: sort space for NaN-holding error vectors
+  if (median) bd.add(ctx->med, sizeof(float) * 2 * (size_t)n * pb.cpad * num_k, &w.med);
+  if (candidate_errors) bd.add(ctx->cand_err, sizeof(float) * (size_t)d * d, &w.cand_err);
+  rc = bd.bind();
+  if (rc) return rc;
+  if (timed) {
+    rc = ensure_events(ctx);
+    if (rc) return rc;
   }
 
   hipStream_t s = ctx->stream;
-  HIPCHK(hipMemcpyAsync(ctx->kvals.p, k_vals, sizeof(int) * num_k, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(ctx->losses.p, loss_functions, sizeof(int) * num_loss,
-                        hipMemcpyHostToDevice, s));
-  if (cand_err) {
-    HIPCHK(snn::launch_fill(cand_err, (int64_t)d * d, std::nanf(""), s));
+  HIPCHK(hipMemcpyAsync(w.kvals, k_vals, sizeof(int) * num_k, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(w.losses, loss_functions, sizeof(int) * num_loss, hipMemcpyHostToDevice,
+                        s));
+  if (w.cand_err) HIPCHK(snn::launch_fill(w.cand_err, (int64_t)d * d, std::nanf(""), s));
+  if (pruned) {
+    HIPCHK(snn::launch_fill_u64(w.visited, 1, s));
+    HIPCHK(snn::launch_fill(w.tauw, (int64_t)n * d, snn::kMaxF, s));  // no warm bound yet
+    if (w.lcount) HIPCHK(snn::launch_fill_i32(w.lcount, n, -1, s));   // no row lists yet
   }
 
-  Timing tm;
-  hipEvent_t e0 = tm.make(), e1 = tm.make();
-  HIPCHK(hipEventRecord(e0, s));
-  SelState* st = ctx->state.as<SelState>();
-  float* D = ctx->D.as<float>();
-  float* Dlo = comp ? ctx->Dlo.as<float>() : nullptr;
-  HIPCHK(snn::launch_transpose(dX, ctx->XT.as<float>(), n, d, s));
+  EventPool& ep = ctx->events;
+  hipEvent_t e0 = timed ? ep.span[0] : nullptr, e1 = timed ? ep.span[1] : nullptr;
+  hipEvent_t e2 = timed ? ep.span[2] : nullptr;
+  if (timed) HIPCHK(hipEventRecord(e0, s));
+  SelState* st = w.st;
+  float* D = w.D;
+  float* Dlo = w.Dlo;
+  HIPCHK(snn::launch_transpose(dX, w.XT, n, d, s));
   // candidate slots: ascending feature index (plain), grouped by scale (pruned)
-  HIPCHK(snn::launch_feature_order(ctx->XT.as<float>(), n, d, ctx->key.as<double>(),
-                                   ctx->perm.as<int>(), pruned, s));
-  HIPCHK(snn::launch_init_state(st, pb, ctx->cand.as<int>(), ctx->sel.as<int>(),
-                                ctx->ordering.as<int>(), ctx->errors.as<float>(),
-                                ctx->perm.as<int>(), s));
+  HIPCHK(snn::launch_feature_order(w.XT, n, d, w.key, w.perm, w.iperm, pruned, s));
+  HIPCHK(snn::launch_init_state(st, pb, w.cand, w.sel, w.ordering, w.errors, w.perm, s));
   if (sorted_first)
-    HIPCHK(snn::launch_sort_columns(ctx->XT.as<float>(), n, d, st, ctx->skey.as<float>(),
-                                    ctx->sidx.as<int>(), ctx->sort_tmp.p, sort_bytes, s));
+    HIPCHK(snn::launch_sort_columns(w.XT, n, d, st, w.skey, w.sidx, w.sort_tmp, sort_bytes, s));
+  else if (pruned)  // the warm bounds need finite distances
+    HIPCHK(snn::launch_check_finite(w.XT, (int64_t)n * d, st, s));
   if (fwd) {
     HIPCHK(snn::launch_fill(D, (int64_t)nn, 0.0f, s));
     if (Dlo) HIPCHK(snn::launch_fill(Dlo, (int64_t)nn, 0.0f, s));
   } else {
-    HIPCHK(snn::launch_gram_exact(ctx->XT.as<float>(), D, Dlo, pruned ? ctx->M.as<float>() : nullptr,
-                                  n, d, s));
+    HIPCHK(snn::launch_gram_exact(w.XT, D, Dlo, pruned ? w.M : nullptr, n, d, s));
   }
-  HIPCHK(hipEventRecord(e1, s));
+  if (timed) HIPCHK(hipEventRecord(e1, s));
 
   snn::SearchProblem sl{};
   sl.D = D;
-  sl.Xc = ctx->Xc.as<float>();
+  sl.Xc = w.Xc;
   sl.y = dy;
-  sl.kvals = ctx->kvals.as<int>();
+  sl.kvals = w.kvals;
   sl.st = st;
-  sl.P = ctx->P.as<float>();
+  sl.P = w.P;
   sl.n = n;
   sl.npf = pb.npf;
   sl.cpad = pb.cpad;
@@ -354,52 +491,69 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   sl.mult = fwd ? 1 : -1;
   sl.model_type = prm->model_type;
   sl.zero = 0;
+  sl.cand = w.cand;
+  sl.d = d;
   if (sorted_first) {
     sl.sorted = 1;
-    sl.skey = ctx->skey.as<float>();
-    sl.sidx = ctx->sidx.as<int>();
-    sl.cand = ctx->cand.as<int>();
+    sl.skey = w.skey;
+    sl.sidx = w.sidx;
   }
   if (pruned) {
-    sl.M = fwd ? nullptr : ctx->M.as<float>();
+    sl.M = fwd ? nullptr : w.M;
     sl.nseed = snn::backward_seed_count(kmax);
-    sl.seeds = ctx->seeds.as<int>();
-    sl.visited = ctx->visited.as<unsigned long long>();
+    sl.seeds = w.seeds;
+    sl.visited = w.visited;
+    sl.tauw = w.tauw;
+    sl.iperm = w.iperm;
+    sl.XT = w.XT;
+    sl.frange = w.frange;
+    if (rows_cap) sl.rows = snn::RowList{w.lidx, w.lval, w.lcount, rows_cap};
   }
 
   // per step: [a] gather [b] bound seeds [b2] search [c] losses + arg-min
-  // [u] update [next a]
-  constexpr int kMarks = 5;
+  // [u] update [next a]; the marks live in the context's event ring
+  static const bool trace = std::getenv("SPEEDYNN_TRACE_STEPS") != nullptr;
+  constexpr int R = EventPool::kRingSteps, MK = EventPool::kMarks;
+  PhaseTimes pt;
   int used_variant = 0;
-  std::vector<hipEvent_t> marks;
   // backward: M is recomputed over the remaining candidates each time their
   // number has halved (d/2, d/4, ...), keeping the bound fl(D - M) tight
   int m_refresh = d / 2;
   // precision form: each update also writes the next step's seeds (rows of a
   // multiple of 4 floats); the first pruned step of a backward run scans them
   const bool fused_seeds = pruned && n % 4 == 0;
+  int issued = 0;
   for (int step = 0; step < pb.max_steps; ++step) {
-    hipEvent_t a = tm.make(), b = tm.make(), b2 = tm.make(), c = tm.make(), u = tm.make();
-    HIPCHK(hipEventRecord(a, s));
-    HIPCHK(snn::launch_compact(dX, ctx->cand.as<int>(), st, ctx->Xc.as<float>(), n, d, pb.cpad, s));
-    HIPCHK(hipEventRecord(b, s));
+    hipEvent_t* mk = nullptr;
+    if (timed) {
+      if (step >= R) {  // the slot's previous step: read its times before reuse
+        rc = retire_step(ctx, step - R, step, nullptr, &pt, trace);
+        if (rc) return rc;
+      }
+      mk = &ep.ring[(size_t)(step % R) * MK];
+    }
+    auto mark = [&](int i) { return timed ? hipEventRecord(mk[i], s) : hipSuccess; };
+    HIPCHK(mark(0));
+    HIPCHK(snn::launch_compact(dX, w.cand, st, w.Xc, n, d, pb.cpad, s));
+    HIPCHK(mark(1));
     const bool pruned_step = pruned && (step > 0 || !fwd);
     if (pruned && !fwd && step > 0 && d - step <= m_refresh) {
-      HIPCHK(snn::launch_max_square(ctx->Xc.as<float>(), st, ctx->M.as<float>(), n, pb.cpad, s));
+      HIPCHK(snn::launch_max_square(w.Xc, st, w.M, n, pb.cpad, s));
       m_refresh = (d - step) / 2;
     }
     // seeds: the previous step's update wrote them where it was fused
     if (pruned_step && !(fused_seeds && step > 0))
-      HIPCHK(snn::launch_seed_scan(D, st, ctx->seeds.as<int>(), n, pb.npf, s));
-    HIPCHK(hipEventRecord(b2, s));
+      HIPCHK(snn::launch_seed_scan(D, st, w.seeds, n, pb.npf, s));
+    HIPCHK(mark(2));
     if (pruned_step) {
       // the second forward step may walk the first chosen feature's sorted column
       const int* rank = nullptr;
       if (sorted_first && step == 1) {
-        HIPCHK(snn::launch_rank(ctx->sidx.as<int>(), st, ctx->rank.as<int>(), n, s));
-        rank = ctx->rank.as<int>();
+        HIPCHK(snn::launch_rank(w.sidx, st, w.rank, n, s));
+        rank = w.rank;
       }
-      int rc2 = search_pruned_tuned(sl, d, step, rank, s, &used_variant);
+      int rc2 = search_pruned_tuned(sl, d, step, rank, s, &used_variant,
+                                    timed ? ep.tune : nullptr);
       if (rc2) return rc2;
     } else {
       // first forward step (D = 0): sorted columns; the dense search runs
@@ -407,42 +561,48 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
       if (sorted_first) HIPCHK(snn::launch_search_sorted(sl, d, s));
       sl.zero = pruned ? 1 : 0;
       HIPCHK(snn::launch_search_dense(sl, s));
+      sl.zero = 0;
     }
-    HIPCHK(hipEventRecord(c, s));
-    HIPCHK(snn::launch_loss(ctx->P.as<float>(), dy, ctx->losses.as<int>(), st,
-                            ctx->err.as<float>(), n, pb.cpad, num_k, num_loss, d, med,
+    HIPCHK(mark(3));
+    HIPCHK(snn::launch_loss(w.P, dy, w.losses, st, w.err, n, pb.cpad, num_k, num_loss, d, w.med,
                             snn::loss_paths(loss_functions, num_loss), s));
-    HIPCHK(snn::launch_select(st, pb, ctx->kvals.as<int>(), ctx->err.as<float>(),
-                              ctx->cand.as<int>(), ctx->sel.as<int>(), ctx->ordering.as<int>(),
-                              ctx->errors.as<float>(), ctx->val.as<float>(), ctx->bestk.as<int>(),
-                              cand_err, ctx->perm.as<int>(), s));
-    HIPCHK(hipEventRecord(u, s));
+    HIPCHK(snn::launch_select(st, pb, w.kvals, w.err, w.cand, w.sel, w.ordering, w.errors, w.val,
+                              w.bestk, w.cand_err, w.perm, s));
+    // the forward warm bound's largest square comes from the chosen column
+    if (pruned && fwd && step + 1 < pb.max_steps)
+      HIPCHK(snn::launch_col_range(w.XT, st, n, w.frange, s));
+    HIPCHK(mark(4));
     // the last step's choice is not applied to D: no later search reads it
     // (the state's pending flag is reset when the next selection starts)
     if (step + 1 < pb.max_steps) {
       if (fused_seeds)
-        HIPCHK(snn::launch_update_seeds(D, Dlo, ctx->XT.as<float>(), n, st, fwd ? 1 : -1,
-                                        ctx->seeds.as<int>(), pb.npf, s));
+        HIPCHK(snn::launch_update_seeds(D, Dlo, w.XT, n, st, fwd ? 1 : -1, w.seeds, pb.npf,
+                                        sl.rows, w.tauw, w.frange, d, w.cand, w.iperm, s));
       else
-        HIPCHK(snn::launch_update(D, Dlo, ctx->XT.as<float>(), n, st, -1, fwd ? 1 : -1, s));
+        HIPCHK(snn::launch_update(D, Dlo, w.XT, n, st, -1, fwd ? 1 : -1, s));
     }
-    for (hipEvent_t e : {a, b, b2, c, u}) marks.push_back(e);
+    issued = step + 1;
   }
-  hipEvent_t e2 = tm.make();
-  HIPCHK(hipEventRecord(e2, s));
+  if (timed) HIPCHK(hipEventRecord(e2, s));
   HIPCHK(hipStreamSynchronize(s));
+  if (timed) {
+    for (int t = issued > R ? issued - R : 0; t < issued; ++t) {
+      rc = retire_step(ctx, t, issued, e2, &pt, trace);
+      if (rc) return rc;
+    }
+  }
 
   SelState hs{};
   HIPCHK(hipMemcpy(&hs, st, sizeof(SelState), hipMemcpyDeviceToHost));
   if (selected_dimensions)
-    HIPCHK(hipMemcpy(selected_dimensions, ctx->sel.p, sizeof(int) * d, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(selected_dimensions, w.sel, sizeof(int) * d, hipMemcpyDeviceToHost));
   if (selected_ordering)
-    HIPCHK(hipMemcpy(selected_ordering, ctx->ordering.p, sizeof(int) * d, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(selected_ordering, w.ordering, sizeof(int) * d, hipMemcpyDeviceToHost));
   if (selected_errors)
-    HIPCHK(hipMemcpy(selected_errors, ctx->errors.p, sizeof(float) * (size_t)num_loss * d,
+    HIPCHK(hipMemcpy(selected_errors, w.errors, sizeof(float) * (size_t)num_loss * d,
                      hipMemcpyDeviceToHost));
   if (candidate_errors)
-    HIPCHK(hipMemcpy(candidate_errors, cand_err, sizeof(float) * (size_t)d * d,
+    HIPCHK(hipMemcpy(candidate_errors, w.cand_err, sizeof(float) * (size_t)d * d,
                      hipMemcpyDeviceToHost));
 
   if (summary) {
@@ -451,22 +611,15 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     summary->optimal_k = hs.best_k;
     summary->steps = hs.step;
     summary->pruned_kernel = used_variant;
-    summary->ms_total = elapsed(e0, e2);
-    summary->ms_distance = elapsed(e0, e1);
-    double ms_search = 0, ms_update = 0, ms_tally = 0, ms_bound = 0;
-    for (size_t i = 0; i + kMarks - 1 < marks.size(); i += kMarks) {
-      ms_tally += elapsed(marks[i], marks[i + 1]) + elapsed(marks[i + 3], marks[i + 4]);
-      ms_bound += elapsed(marks[i + 1], marks[i + 2]);
-      const double sm = elapsed(marks[i + 2], marks[i + 3]);
-      ms_search += sm;
-      if (i == 0) summary->ms_search_first = sm;
-      const hipEvent_t next = (i + kMarks < marks.size()) ? marks[i + kMarks] : e2;
-      ms_update += elapsed(marks[i + 4], next);
+    if (timed) {
+      summary->ms_total = elapsed(e0, e2);
+      summary->ms_distance = elapsed(e0, e1);
+      summary->ms_search = pt.search;
+      summary->ms_update = pt.update;
+      summary->ms_tally = pt.tally;
+      summary->ms_bound = pt.bound;
+      summary->ms_search_first = pt.search_first;
     }
-    summary->ms_search = ms_search;
-    summary->ms_update = ms_update;
-    summary->ms_tally = ms_tally;
-    summary->ms_bound = ms_bound;
     // candidates evaluated per step: fwd d, d-1, ...; bwd d, d-1, ...
     int64_t ev = 0, pruned_full = 0;
     for (int t = 0; t < hs.step; ++t) {
@@ -476,7 +629,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     summary->evals = ev;
     if (pruned) {
       unsigned long long done = 0;
-      HIPCHK(hipMemcpy(&done, ctx->visited.p, sizeof(done), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(&done, w.visited, sizeof(done), hipMemcpyDeviceToHost));
       summary->pruned_point_evals = pruned_full - (int64_t)done;
     }
   }
@@ -563,8 +716,10 @@ void snn_destroy(snn_ctx* c) {
                     &c->bestk, &c->cand, &c->sel, &c->ordering, &c->errors, &c->cand_err,
                     &c->kvals, &c->losses, &c->state, &c->scratch, &c->seeds,
                     &c->visited, &c->perm, &c->key, &c->M, &c->med, &c->skey,
-                    &c->sidx, &c->sort_tmp, &c->rank};
+                    &c->sidx, &c->sort_tmp, &c->rank, &c->tauw, &c->iperm, &c->frange,
+                    &c->lidx, &c->lval, &c->lcount};
   for (DevBuf* b : bufs) b->release();
+  destroy_events(c);
   c->batch.drop_graphs();
   c->batch.arena.release();
   c->batch.meta.release();

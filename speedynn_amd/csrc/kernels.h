@@ -6,6 +6,7 @@ namespace snn {
 
 hipError_t launch_fill(float* p, int64_t count, float v, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, int64_t count, hipStream_t s);
+hipError_t launch_fill_i32(int* p, int64_t count, int v, hipStream_t s);
 hipError_t launch_transpose(const float* X, float* XT, int64_t n, int d, hipStream_t s);
 hipError_t launch_gram_exact(const float* XT, float* D, float* Dlo, float* M, int64_t n, int d,
                              hipStream_t s);
@@ -15,9 +16,12 @@ hipError_t launch_max_square(const float* Xc, const SelState* st, float* M, int6
                              hipStream_t s);
 // The update of the device state's pending feature fused with the seed scan
 // of the next step (seeds as launch_seed_scan writes them); n % 4 == 0.
+// Forward: rl.idx set also writes the next search's row lists (RowList) from
+// the warm bounds tauw and the chosen column's range frange.
 hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
                                const SelState* st, int mult, int* seeds, int64_t npf,
-                               hipStream_t s);
+                               const RowList& rl, const float* tauw, const float* frange, int d,
+                               const int* cand, const int* iperm, hipStream_t s);
 hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const SelState* st,
                          int f, int mult, hipStream_t s);
 hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* Xc, int64_t n,
@@ -67,8 +71,14 @@ bool needs_median_scratch(const int* codes, int count);
 hipError_t launch_select(SelState* st, const Problem& pb, const int* kvals, const float* err,
                          int* cand, int* sel, int* ordering, float* errors, float* val,
                          int* bestk, float* cand_err, const int* perm, hipStream_t s);
+// perm[rank] = feature, iperm[feature] = rank (iperm may be null).
 hipError_t launch_feature_order(const float* XT, int64_t n, int d, double* key, int* perm,
-                                bool by_scale, hipStream_t s);
+                                int* iperm, bool by_scale, hipStream_t s);
+// frange = {min, max} of XT's column of the device state's last choice.
+hipError_t launch_col_range(const float* XT, const SelState* st, int64_t n, float* frange,
+                            hipStream_t s);
+// st->nonfinite |= (some of the count values is NaN or infinite).
+hipError_t launch_check_finite(const float* XT, int64_t count, SelState* st, hipStream_t s);
 hipError_t launch_init_state(SelState* st, const Problem& pb, int* cand, int* sel, int* ordering,
                              float* errors, const int* perm, hipStream_t s);
 

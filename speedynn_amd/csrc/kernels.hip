@@ -31,6 +31,11 @@ __global__ void k_fill(float* __restrict__ p, int64_t count, float v) {
   for (int64_t t = (n4 << 2) + i; t < count; t += stride) p[t] = v;
 }
 
+__global__ void k_fill_i32(int* __restrict__ p, int64_t count, int v) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = v;
+}
+
 __global__ void k_zero_u64(unsigned long long* __restrict__ p, int64_t count) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += stride) p[t] = 0;
@@ -299,14 +304,25 @@ __global__ __launch_bounds__(256) void k_update(float* __restrict__ D, float* __
 // then written ranked by D, as k_seed_scan writes them.  Saves the seed
 // scan's full read of D on every pruned step.
 template <bool COMP>
+//
+// Forward (rl.idx set): the row's list for the next search (RowList) is
+// written as well.  Its bound is the largest warm bound over the next step's
+// candidates (cand[c < ncand], k_select has listed them): fl(max tau + Q)
+// widened as in the search's warm bound; no list where some candidate's tau
+// is unknown (kMaxF) or X is not finite.
 __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
                                                       float* __restrict__ Dlo,
                                                       const float* __restrict__ XT, int64_t n,
                                                       const SelState* __restrict__ st, int mult,
-                                                      int* __restrict__ seeds, int64_t npf) {
+                                                      int* __restrict__ seeds, int64_t npf,
+                                                      RowList rl, const float* __restrict__ tauw,
+                                                      const float* __restrict__ frange, int d,
+                                                      const int* __restrict__ cand,
+                                                      const int* __restrict__ iperm) {
   if (!st->pending) return;
   const int f = st->fstar;
   const int lane = threadIdx.x & 63;
+  const float kInf = __int_as_float(0x7f800000);
   const float* __restrict__ xf = XT + (int64_t)f * n;
   for (int64_t i = (int64_t)blockIdx.x * 4 + uniform(threadIdx.x >> 6); i < n;
        i += (int64_t)gridDim.x * 4) {
@@ -317,21 +333,80 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
     const int64_t fhi = flo + npf < n ? flo + npf : n;
     SeedPair sp;
     sp.reset();
-    auto upd = [&](v4f& dv, v4f& lv, const v4f xj, int64_t j) {
+    // the row list's bound
+    float Tq = -kInf;  // -inf: no list
+    if (rl.idx && !st->nonfinite) {
+      float m = 0.0f;
+      bool unknown = false;
+      const int nc = st->ncand;
+      for (int c = lane; c < nc; c += 64) {
+        const float t = tauw[i * d + iperm[cand[c]]];
+        unknown |= !(t < kMaxF);
+        m = fmaxf(m, t);
+      }
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const float t = xi - xj[e];
-        const float p = t * t;
-        const float ps = mult > 0 ? p : -p;
-        const float s = dv[e] + ps;
-        if (COMP) lv[e] += comp_error(xi, xj[e], t, p, dv[e], ps, s, mult < 0);
-        dv[e] = s;
-        sp.offer(s, j + e, flo, fhi);
+      for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+      if (!__any(unknown)) {
+        const float lo = xi - frange[0], hi = frange[1] - xi;
+        const float w = fmaf(m + fmaxf(lo * lo, hi * hi), 1.00000095367431640625f, 1.0e-36f);
+        if (w < kInf) Tq = w;
+      }
+    }
+    int64_t lcount = 0;  // list entries written (wave-uniform)
+    int* __restrict__ lidx = rl.idx ? rl.idx + i * rl.cap : nullptr;
+    float* __restrict__ lval = rl.idx ? rl.val + i * rl.cap : nullptr;
+    // in: the lane holds four points of the row (all lanes call this together)
+    auto upd = [&](v4f& dv, v4f& lv, const v4f xj, int64_t j, bool in) {
+      int keep = 0;
+      if (in) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float t = xi - xj[e];
+          const float p = t * t;
+          const float ps = mult > 0 ? p : -p;
+          const float s = dv[e] + ps;
+          if (COMP) lv[e] += comp_error(xi, xj[e], t, p, dv[e], ps, s, mult < 0);
+          dv[e] = s;
+          sp.offer(s, j + e, flo, fhi);
+          keep |= (s <= Tq && (j + e < flo || j + e >= fhi)) ? 1 << e : 0;
+        }
+      }
+      if (Tq != -kInf) {
+        // entries in ascending index: lane L's four points precede lane L+1's
+        const int c = __builtin_popcount(keep);
+        int pre = 0, tot = 0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          const unsigned long long m = __ballot((c >> b) & 1);
+          pre += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << b;
+          tot += (int)__popcll(m) << b;
+        }
+        if (lcount + tot > rl.cap) {
+          Tq = -kInf;  // too many: no list for this row
+          lcount = -1;
+        } else {
+          int pos = (int)lcount + pre;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            if (keep & (1 << e)) {
+              lidx[pos] = (int)(j + e);
+              lval[pos] = dv[e];
+              ++pos;
+            }
+          }
+          lcount += tot;
+        }
       }
     };
+    // every lane runs the same iterations (the list's ballots need the whole
+    // wave): blocks of U * 256 points, then 256-point chunks in which lanes
+    // past the row's end hold no point (n % 4 == 0: a lane's four points are
+    // all in the row or all past it)
     constexpr int U = 4;
-    int64_t j = (int64_t)lane * 4;
-    for (; j + (U - 1) * 256 < n; j += U * 256) {
+    int64_t jb = 0;
+    for (; jb + U * 256 <= n; jb += U * 256) {
+      const int64_t j = jb + (int64_t)lane * 4;
       v4f dv[U], lv[U], xj[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -341,21 +416,28 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        upd(dv[u], lv[u], xj[u], j + u * 256);
+        upd(dv[u], lv[u], xj[u], j + u * 256, true);
         __builtin_nontemporal_store(dv[u], reinterpret_cast<v4f*>(row + j + u * 256));
         if (COMP) *reinterpret_cast<v4f*>(rlo + j + u * 256) = lv[u];
       }
     }
-    for (; j < n; j += 256) {
-      v4f lv = {0.f, 0.f, 0.f, 0.f};
-      const v4f xj = *reinterpret_cast<const v4f*>(xf + j);
-      v4f dv = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j));
-      if (COMP) lv = *reinterpret_cast<v4f*>(rlo + j);
-      upd(dv, lv, xj, j);
-      __builtin_nontemporal_store(dv, reinterpret_cast<v4f*>(row + j));
-      if (COMP) *reinterpret_cast<v4f*>(rlo + j) = lv;
+    for (; jb < n; jb += 256) {
+      const int64_t j = jb + (int64_t)lane * 4;
+      const bool in = j < n;
+      v4f lv = {0.f, 0.f, 0.f, 0.f}, xj = {0.f, 0.f, 0.f, 0.f}, dv = {0.f, 0.f, 0.f, 0.f};
+      if (in) {
+        xj = *reinterpret_cast<const v4f*>(xf + j);
+        dv = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j));
+        if (COMP) lv = *reinterpret_cast<v4f*>(rlo + j);
+      }
+      upd(dv, lv, xj, j, in);
+      if (in) {
+        __builtin_nontemporal_store(dv, reinterpret_cast<v4f*>(row + j));
+        if (COMP) *reinterpret_cast<v4f*>(rlo + j) = lv;
+      }
     }
     sp.write_ranked(seeds + i * 128);
+    if (rl.idx && lane == 0) rl.count[i] = Tq == -kInf ? -1 : (int)lcount;
   }
 }
 
@@ -966,11 +1048,12 @@ __global__ __launch_bounds__(256) void k_feature_key(const float* __restrict__ X
   if (threadIdx.x == 0) key[blockIdx.x] = s2[0] - s1[0] * s1[0] / (double)n;
 }
 
-// perm[rank(f)] = f, rank by (descending key, index); identity when key is
-// null.  A NaN key (a column holding NaN or infinities) ranks as +inf, so the
-// ranks stay a permutation.
+// perm[rank(f)] = f and iperm[f] = rank(f) (iperm may be null), rank by
+// (descending key, index); identity when key is null.  A NaN key (a column
+// holding NaN or infinities) ranks as +inf, so the ranks stay a permutation.
 __global__ __launch_bounds__(256) void k_feature_rank(const double* __restrict__ key, int d,
-                                                      int* __restrict__ perm) {
+                                                      int* __restrict__ perm,
+                                                      int* __restrict__ iperm) {
   const double kInf = __longlong_as_double(0x7ff0000000000000LL);
   for (int f = threadIdx.x; f < d; f += blockDim.x) {
     int r = f;
@@ -983,6 +1066,42 @@ __global__ __launch_bounds__(256) void k_feature_rank(const double* __restrict__
       }
     }
     perm[r] = f;
+    if (iperm) iperm[f] = r;
+  }
+}
+
+// frange = {min, max} of the column of the feature the last step chose (the
+// forward warm bound's largest square); one block.
+__global__ __launch_bounds__(1024) void k_col_range(const float* __restrict__ XT,
+                                                    const SelState* __restrict__ st, int64_t n,
+                                                    float* __restrict__ frange) {
+  __shared__ float s_lo[16], s_hi[16];
+  const int g = st->fstar;
+  if (st->stopped || g < 0) return;
+  const float* __restrict__ col = XT + (int64_t)g * n;
+  float lo = __int_as_float(0x7f800000), hi = -lo;
+  for (int64_t j = threadIdx.x; j < n; j += 1024) {
+    const float v = col[j];
+    lo = fminf(lo, v);
+    hi = fmaxf(hi, v);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fminf(lo, __shfl_xor(lo, o));
+    hi = fmaxf(hi, __shfl_xor(hi, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_lo[threadIdx.x >> 6] = lo;
+    s_hi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 16; ++w) {
+      lo = fminf(lo, s_lo[w]);
+      hi = fmaxf(hi, s_hi[w]);
+    }
+    frange[0] = fminf(lo, s_lo[0]);
+    frange[1] = fmaxf(hi, s_hi[0]);
   }
 }
 
@@ -1020,6 +1139,11 @@ static inline int grid_1d(int64_t work, int block, int cap) {
   int64_t g = (work + block - 1) / block;
   if (g < 1) g = 1;
   return (int)(g > cap ? cap : g);
+}
+
+hipError_t launch_fill_i32(int* p, int64_t count, int v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_1d(count, 256, 1024)), dim3(256), 0, s, p, count, v);
+  return hipGetLastError();
 }
 
 hipError_t launch_fill_u64(unsigned long long* p, int64_t count, hipStream_t s) {
@@ -1072,16 +1196,17 @@ hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const
 
 hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
                                const SelState* st, int mult, int* seeds, int64_t npf,
-                               hipStream_t s) {
+                               const RowList& rl, const float* tauw, const float* frange, int d,
+                               const int* cand, const int* iperm, hipStream_t s) {
   if (n % 4) return hipErrorInvalidValue;
   const int64_t rows = (n + 3) / 4;
   const int g = (int)(rows < 8192 ? rows : 8192);
   if (Dlo)
     hipLaunchKernelGGL((k_update_seeds<true>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st, mult,
-                       seeds, npf);
+                       seeds, npf, rl, tauw, frange, d, cand, iperm);
   else
     hipLaunchKernelGGL((k_update_seeds<false>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st, mult,
-                       seeds, npf);
+                       seeds, npf, rl, tauw, frange, d, cand, iperm);
   return hipGetLastError();
 }
 
@@ -1136,10 +1261,16 @@ hipError_t launch_select(SelState* st, const Problem& pb, const int* kvals, cons
 }
 
 hipError_t launch_feature_order(const float* XT, int64_t n, int d, double* key, int* perm,
-                                bool by_scale, hipStream_t s) {
+                                int* iperm, bool by_scale, hipStream_t s) {
   if (by_scale) hipLaunchKernelGGL(k_feature_key, dim3(d), dim3(256), 0, s, XT, n, key);
   hipLaunchKernelGGL(k_feature_rank, dim3(1), dim3(256), 0, s, by_scale ? key : nullptr, d,
-                     perm);
+                     perm, iperm);
+  return hipGetLastError();
+}
+
+hipError_t launch_col_range(const float* XT, const SelState* st, int64_t n, float* frange,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_col_range, dim3(1), dim3(1024), 0, s, XT, st, n, frange);
   return hipGetLastError();
 }
 

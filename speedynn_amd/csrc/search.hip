@@ -9,6 +9,14 @@
 #include "common.h"
 #include "kernels.h"
 
+// Waves per SIMD the forward per-wave pruned search is compiled for (the
+// register budget of its body): lists up to 16 fit six waves (80 registers);
+// the 20-entry list runs five (96 registers: at 80 it spills, which cost the
+// n = 65536 forward selection 40% of its late-step search time).
+#ifndef SNN_PRUNED_WAVES
+#define SNN_PRUNED_WAVES(K) ((K) <= 16 ? 6 : 5)
+#endif
+
 namespace snn {
 
 // ------------------------------------------------------ (c) candidate search
@@ -232,12 +240,46 @@ __device__ __forceinline__ float screen_limit(float tau) {
   return MULT > 0 ? fmaf(tau, 1.00000095367431640625f, 1.0e-36f) : next_up(tau);
 }
 
+// Warm bound of candidate rank r for query q, from the K-th distance tau of
+// the last step that searched it (+inf where none is known or X is not
+// finite).  The K points of that list are eligible and distinct, so their
+// distances now bound the new K-th distance:
+//   backward: D only loses terms, D' = fl(D - q*) <= D, and by monotone
+//     rounding fl(D' - t^2) <= fl(D - t^2) <= tau: tau itself is a bound;
+//   forward: D' = fl(D + q*) with q* = fl((x_qg - x_jg)^2) <= Q, the square of
+//     the feature g chosen since, largest over all j (from the column's min
+//     and max; rounding is monotone), so fl(D' + t^2) <= (tau + Q)(1 + 3.01u)
+//     (u = 2^-24; fl(D + t^2) <= tau gives D + t^2 <= tau(1 + u)): the bound
+//     is fl(tau + Q) widened by 2^-20, plus 1e-36 for the subnormal range.
+template <int MULT>
+__device__ __forceinline__ float warm_bound(const SearchProblem& a, int64_t q, int r) {
+  const float kInf = __int_as_float(0x7f800000);
+  if (!a.tauw || a.st->nonfinite) return kInf;
+  const float tau = a.tauw[q * a.d + r];
+  if (!(tau < kMaxF)) return kInf;
+  if (MULT < 0) return tau;
+  const int g = a.st->fstar;
+  if (g < 0) return kInf;
+  const float xg = a.XT[(int64_t)g * a.n + q];
+  const float lo = xg - a.frange[0], hi = a.frange[1] - xg;
+  const float qm = fmaxf(lo * lo, hi * hi);
+  const float w = fmaf(tau + qm, 1.00000095367431640625f, 1.0e-36f);
+  return w < kInf ? w : kInf;
+}
+
+// Rank (iperm) of the feature in candidate slot c, 0 for padding slots.
+__device__ __forceinline__ int cand_rank(const SearchProblem& a, int c, bool live) {
+  return (a.tauw && live) ? a.iperm[a.cand[c]] : 0;
+}
+
 // Bound of one lane (candidate): the K-th smallest exact distance over the
 // query's seeds, which k_seed_scan lists by ascending D.  Forward distances
 // are >= D, so once the next seed's D exceeds every live lane's current K-th
 // value no later seed can lower a bound, and the scan stops; a backward
-// selection reads the first a.nseed seeds.  Returns +inf for an unfilled list,
-// -inf for padding lanes.
+// selection reads the first a.nseed seeds.  `warm` is a bound known already
+// (warm_bound, +inf if none): the result is the smaller, and a forward scan
+// also stops at a seed whose D exceeds every live lane's warm bound.  Returns
+// +inf for an unfilled list without a warm bound, -inf for padding lanes.
 //
 // PIPE: batches of 8 seeds in two register sets, the next batch's index, D
 // and value loads in flight while this batch's distances are inserted (up to
@@ -245,7 +287,7 @@ __device__ __forceinline__ float screen_limit(float tau) {
 template <int K, int MULT, bool PIPE = true>
 __device__ __forceinline__ float seed_bound(const SearchProblem& a, const float* __restrict__ Xl,
                                             const float* __restrict__ Dq, float xq, int64_t q,
-                                            int cpad, bool live) {
+                                            int cpad, bool live, float warm) {
   const float kInf = __int_as_float(0x7f800000);
   float B[K];
 #pragma unroll
@@ -273,11 +315,11 @@ __device__ __forceinline__ float seed_bound(const SearchProblem& a, const float*
       }
       if (MULT > 0 && s0 + 8 < nseed) {
         const int jn = sq[s0 + 8];
-        const float bmax = wave_max(live ? B[K - 1] : -kInf);
+        const float bmax = wave_max(live ? fminf(B[K - 1], warm) : -kInf);
         if (jn < 0 || Dq[jn] > bmax) break;  // seeds ascend in D; missing ones come last
       }
     }
-    return live ? B[K - 1] : -kInf;
+    return live ? fminf(B[K - 1], warm) : -kInf;
   }
   int js[2][8];
   float xs[2][8], dqs[2][8];
@@ -304,7 +346,7 @@ __device__ __forceinline__ float seed_bound(const SearchProblem& a, const float*
   // at a seed whose D exceeds every live lane's current K-th value
   auto stop = [&](const int (&j_)[8], const float (&d_)[8]) {
     if (MULT < 0) return false;
-    const float bmax = wave_max(live ? B[K - 1] : -kInf);
+    const float bmax = wave_max(live ? fminf(B[K - 1], warm) : -kInf);
     return j_[0] < 0 || d_[0] > bmax;
   };
   load(0, js[0], xs[0], dqs[0]);
@@ -320,7 +362,7 @@ __device__ __forceinline__ float seed_bound(const SearchProblem& a, const float*
     insert(js[1], xs[1], dqs[1]);
     if (!more2 || stop(js[0], dqs[0])) break;
   }
-  return live ? B[K - 1] : -kInf;
+  return live ? fminf(B[K - 1], warm) : -kInf;
 }
 
 // Top-K insertion ordered by (distance, index) whatever the offering order:
@@ -402,7 +444,9 @@ __device__ __forceinline__ float flush_pending(const float* buf, int& pc, float 
 //
 // Bound.  For a candidate f, the K-th smallest d_qjf over any K distinct
 // eligible points is an upper bound tau_f of its final K-th distance; here it
-// is the K-th smallest over the 128 seeds of k_seed_scan, evaluated exactly.
+// is the smaller of the K-th smallest over the 128 seeds of k_seed_scan,
+// evaluated exactly, and the warm bound from the previous step's K-th
+// distance (warm_bound).
 // Each point has a lower bound lb_qj of d_qjf valid for every candidate:
 //   forward  (MULT = +1): d = fl(D + fl(t^2)) >= D, so lb = D;
 //   backward (MULT = -1): M_qj >= fl(t^2) for every feature (max single-
@@ -414,7 +458,9 @@ __device__ __forceinline__ float flush_pending(const float* buf, int& pc, float 
 // so every list equals the dense search's bit for bit.
 //
 // Per 64 points the wave takes the ballot of lb <= T outside the query's fold
-// (the D/M values of the next four chunks are already in flight), then for
+// (the D/M values of the next four chunks are already in flight; a forward
+// step whose update wrote the row's list (RowList) scans that list instead of
+// the row: the same points that can pass, in the same order), then for
 // each such point, ascending, gathers the candidates' values Xc[j][64cb+lane]
 // (256 B, coalesced) and evaluates d.  Forward tests use a screen fma(t,t,D)
 // against the K-th distance widened by 2^-20 (plus 1e-36 for the subnormal
@@ -427,7 +473,7 @@ __device__ __forceinline__ float flush_pending(const float* buf, int& pc, float 
 // may overflow and at the end; the bound and screen limit tighten at each
 // flush.
 // No barriers.  Registers decide the speed: the search is latency-bound, and
-// at K = 20 this body fits six waves per SIMD.
+// at K = 20 this body runs five waves per SIMD (SNN_PRUNED_WAVES).
 template <int K, int MULT, bool PIPE = true>
 __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, int64_t q,
                                             int cb) {
@@ -449,7 +495,9 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
   const float kInf = __int_as_float(0x7f800000);
   auto dist = [](float dq, float p) { return MULT > 0 ? dq + p : dq - p; };
 
-  float thr = seed_bound<K, MULT, PIPE>(a, Xl, Dq, xq, q, cpad, live);
+  const int rk = cand_rank(a, c, live);
+  float thr = seed_bound<K, MULT, PIPE>(a, Xl, Dq, xq, q, cpad, live,
+                                        live ? warm_bound<MULT>(a, q, rk) : kInf);
   float T = wave_max(thr);
 
   float Ld[K];
@@ -479,15 +527,10 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
   auto ldM = [&](int j) {
     return MULT < 0 ? __builtin_nontemporal_load(Mq + (j < n32 ? j : n32 - 1)) : 0.0f;
   };
-  float dsl[4], msl[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    dsl[u] = ldD(64 * u + lane);
-    msl[u] = ldM(64 * u + lane);
-  }
   const uint32_t ucpad = (uint32_t)cpad;  // n * cpad < 2^31 (checked on the host)
-  // the visits of one chunk: the points of `mask`, ascending
-  auto visit = [&](int j0, float dcur, unsigned long long mask) {
+  // the visits of one chunk: the points of `mask`, ascending; jof(b) is the
+  // point index of lane b
+  auto visit = [&](auto jof, float dcur, unsigned long long mask) {
     while (mask) {
       constexpr int BATCH = 8;
       int js[BATCH];
@@ -499,7 +542,7 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
         if (mask) {
           const int b = __ffsll((long long)mask) - 1;
           mask &= mask - 1;
-          js[u] = j0 + b;
+          js[u] = jof(b);
           dqs[u] = lane_value(dcur, b);
           ts[u] = Xl[(uint32_t)js[u] * ucpad];
           cnt = u + 1;
@@ -530,6 +573,44 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
       }
     }
   };
+  const int lcnt = (MULT > 0 && a.rows.idx) ? a.rows.count[q] : -1;
+  if (lcnt >= 0) {
+    // the row's list (RowList): its entries ascend in point index, lie
+    // outside q's fold and hold every point with D <= T; the same chunked
+    // scan with two chunks of 64 entries in flight
+    const int* __restrict__ li = a.rows.idx + q * a.rows.cap;
+    const float* __restrict__ lv = a.rows.val + q * a.rows.cap;
+    const int last = lcnt > 0 ? lcnt - 1 : 0;
+    auto ldI = [&](int e) { return __builtin_nontemporal_load(li + (e < lcnt ? e : last)); };
+    auto ldV = [&](int e) { return __builtin_nontemporal_load(lv + (e < lcnt ? e : last)); };
+    int jsl[2];
+    float dsl[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      jsl[u] = ldI(64 * u + lane);
+      dsl[u] = ldV(64 * u + lane);
+    }
+    for (int g0 = 0; g0 < lcnt; g0 += 128) {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int e = g0 + 64 * u + lane;
+        const unsigned long long mask = __ballot(e < lcnt && dsl[u] <= T);
+        if (mask) {
+          seen += __popcll(mask);
+          const int jv = jsl[u];
+          visit([&](int b) { return __builtin_amdgcn_readlane(jv, b); }, dsl[u], mask);
+        }
+        jsl[u] = ldI(e + 128);
+        dsl[u] = ldV(e + 128);
+      }
+    }
+  } else {
+  float dsl[4], msl[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    dsl[u] = ldD(64 * u + lane);
+    msl[u] = ldM(64 * u + lane);
+  }
   const int flo32 = (int)flo, fhi32 = (int)fhi;
   for (int g0 = 0; g0 < n32; g0 += 256) {
 #pragma unroll
@@ -540,7 +621,8 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
       const unsigned long long mask = __ballot(ok);
       if (mask) {
         seen += __popcll(mask);
-        visit(g0 + 64 * u, dsl[u], mask);
+        const int j0 = g0 + 64 * u;
+        visit([=](int b) { return j0 + b; }, dsl[u], mask);
       }
       // the slot's next chunk, issued once its value is dead, so the load
       // lands in the slot's own register
@@ -548,11 +630,13 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
       if (MULT < 0) msl[u] = ldM(jl + 256);
     }
   }
+  }
   flush();
   if (a.visited && lane == 0) {
     const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
     atomicAdd(a.visited, seen * (unsigned long long)nlive);  // point evaluations performed
   }
+  if (a.tauw && live) a.tauw[q * a.d + rk] = Ld[K - 1];  // the next step's warm bound
   finish_query<K>(a, Li, buf, q, c, live);
 }
 
@@ -638,7 +722,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_search_dense(const SearchProblem
 // M) row once into L2.
 template <int K, int MULT>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(K <= 20 && MULT > 0 ? 6 : 1)))  // forward: <= 80 VGPRs
+__attribute__((amdgpu_waves_per_eu(K <= 20 && MULT > 0 ? SNN_PRUNED_WAVES(K) : 1)))
 void k_search_pruned(const SearchProblem a, int cb_major) {
   __shared__ float lds[4 * wave_lds_words<K>()];  // pending buffers, then epilogue labels
   const int wave = uniform(threadIdx.x >> 6);
@@ -708,7 +792,10 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
   auto dist = [](float dq, float p) { return MULT > 0 ? dq + p : dq - p; };
   const float xq = active ? Xl[(int64_t)q * cpad] : 0.0f;
 
-  float thr = active ? seed_bound<K, MULT>(a, Xl, Dq, xq, q, cpad, live) : -kInf;
+  const int rk = cand_rank(a, c, live);
+  float thr = active ? seed_bound<K, MULT>(a, Xl, Dq, xq, q, cpad, live,
+                                           live ? warm_bound<MULT>(a, q, rk) : kInf)
+                     : -kInf;
   float T = wave_max(thr);
   if (lane == 0) Tw[wave] = T;
 
@@ -844,6 +931,7 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
     const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
     atomicAdd(a.visited, seen * (unsigned long long)nlive);
   }
+  if (a.tauw && live) a.tauw[(int64_t)q * a.d + rk] = Ld[K - 1];
   // epilogue: the labels go to the wave's own pending area (empty after the
   // last flush) when it holds K slots, so all waves finish at once; longer
   // lists take four waves at a time through the label area (aliases the queue)
@@ -917,8 +1005,10 @@ __global__ __launch_bounds__(256) void k_search_walk(const SearchProblem a,
     Ld[m] = kMaxF;
     Li[m] = 0;
   }
-  float lim = live ? kInf : -kInf;
-  float T = kInf;
+  const int rk = cand_rank(a, c, live);
+  const float warm = live ? warm_bound<1>(a, q, rk) : -kInf;  // per lane; -inf: padding
+  float lim = screen_limit<1>(warm);
+  float T = wave_max(warm);
   unsigned long long seen = 0;
   // Windows of 64 sorted positions, one lane per position: left windows end
   // at lo - 1 (lane 63 nearest the query), right windows start at hi (lane 0
@@ -985,20 +1075,21 @@ __global__ __launch_bounds__(256) void k_search_walk(const SearchProblem a,
               const float dd = dqs[u] + ts[u] * ts[u];
               if (pair_less(dd, js[u], Ld[K - 1], Li[K - 1])) {
                 topk_insert_pair<K>(Ld, Li, dd, js[u]);
-                lim = Ld[K - 1] < kMaxF ? fmaf(Ld[K - 1], kScreen, kTiny) : kInf;
+                if (Ld[K - 1] < kMaxF) lim = fminf(lim, fmaf(Ld[K - 1], kScreen, kTiny));
                 changed = true;
               }
             }
           }
         }
       }
-      if (__ballot(changed)) T = wave_max(live ? Ld[K - 1] : -kInf);
+      if (__ballot(changed)) T = wave_max(live ? fminf(Ld[K - 1], warm) : -kInf);
     }
   }
   if (a.visited && lane == 0) {
     const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
     atomicAdd(a.visited, seen * (unsigned long long)nlive);
   }
+  if (a.tauw && live) a.tauw[q * a.d + rk] = Ld[K - 1];
   finish_query<K>(a, Li, lab, q, c, live);
 }
 

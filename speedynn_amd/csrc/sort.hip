@@ -63,12 +63,16 @@ size_t sort_columns_temp_bytes(int64_t n, int d) {
   return bytes;
 }
 
+hipError_t launch_check_finite(const float* XT, int64_t count, SelState* st, hipStream_t s) {
+  const int grid = (int)((count + 255) / 256 < 4096 ? (count + 255) / 256 : 4096);
+  hipLaunchKernelGGL(k_check_finite, dim3(grid), dim3(256), 0, s, XT, count, st);
+  return hipGetLastError();
+}
+
 hipError_t launch_sort_columns(const float* XT, int64_t n, int d, SelState* st, float* skey,
                                int* sidx, void* temp, size_t temp_bytes, hipStream_t s) {
   const int64_t count = n * d;
-  const int grid = (int)((count + 255) / 256 < 4096 ? (count + 255) / 256 : 4096);
-  hipLaunchKernelGGL(k_check_finite, dim3(grid), dim3(256), 0, s, XT, count, st);
-  hipError_t e = hipGetLastError();
+  hipError_t e = launch_check_finite(XT, count, st, s);
   if (e != hipSuccess) return e;
   const unsigned int un = (unsigned int)n;
   auto offs = offset_iter(un);
