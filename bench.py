@@ -41,39 +41,6 @@ METRIC = "forward-selection wall s + (query x candidate) kNN evals/s; % HBM roof
 LIST_SIZES = [2, 4, 5, 8, 10, 15, 16, 20, 24, 32, 50, 64]
 
 
-class Hip:
-    """Minimal libamdhip64 binding for device-resident inputs."""
-
-    def __init__(self):
-        self.L = C.CDLL("libamdhip64.so")
-        self.L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-        self.L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-        self.L.hipFree.argtypes = [C.c_void_p]
-        self.L.hipDeviceSynchronize.argtypes = []
-
-    def alloc(self, nbytes: int) -> int:
-        p = C.c_void_p()
-        if self.L.hipMalloc(C.byref(p), nbytes) != 0:
-            raise RuntimeError("hipMalloc failed")
-        return p.value
-
-    def upload(self, a: np.ndarray) -> int:
-        p = self.alloc(a.nbytes)
-        if self.L.hipMemcpy(C.c_void_p(p), a.ctypes.data, a.nbytes, 1) != 0:
-            raise RuntimeError("hipMemcpy failed")
-        return p
-
-    def copy_d2d(self, dst: int, src: int, nbytes: int) -> None:
-        if self.L.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), nbytes, 3) != 0:
-            raise RuntimeError("hipMemcpy failed")
-
-    def free(self, p: int) -> None:
-        self.L.hipFree(C.c_void_p(p))
-
-    def sync(self) -> None:
-        self.L.hipDeviceSynchronize()
-
-
 def list_size(kmax: int) -> int:
     return next(v for v in LIST_SIZES if kmax <= v)
 
@@ -377,7 +344,8 @@ def main() -> int:
               "engine runs on a single device.", file=sys.stderr)
         return 2
 
-    hip = Hip()
+    from speedynn_amd._lib import DeviceMemory
+    hip = DeviceMemory()
     if args.copy_calibration:
         a, b = hip.alloc(args.copy_calibration), hip.alloc(args.copy_calibration)
         hip.copy_d2d(b, a, args.copy_calibration)

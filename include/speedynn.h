@@ -143,6 +143,27 @@ int snn_select_device(snn_ctx *ctx, const float *X_device, int64_t n, int d,
                       int *selected_dimensions, int *selected_ordering, float *selected_errors,
                       float *candidate_errors, snn_summary *summary);
 
+/* Bytes of the fixed device work array of snn_select_device_noalloc (one per GPU). */
+#define SNN_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* snn_select_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Every device work buffer (distance matrix, candidate columns, seeds, ...)
+ * is a range of one fixed array of SNN_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends; a problem that needs more returns SNN_EINVAL with
+ * the size it needs.  Inputs and outputs are as in snn_select_device.  The
+ * summary's times stay 0 (no events are recorded) and no autotuner trials
+ * run (kernels the process has already chosen for the shape, else the
+ * defaults).  Calls on one GPU share the array: calls that overlap on one
+ * GPU race. */
+int snn_select_device_noalloc(snn_ctx *ctx, const float *X_device, int64_t n, int d,
+                              const float *y_device, const int *k_vals, int num_k,
+                              const int *loss_functions, int num_loss, const snn_params *params,
+                              int *selected_dimensions, int *selected_ordering,
+                              float *selected_errors, float *candidate_errors,
+                              snn_summary *summary);
+
 /* One selection problem of a batch: host inputs and host outputs. */
 typedef struct {
     const float *X;              /* n x d row-major */

@@ -24,8 +24,9 @@ EXPORTS = [
     "get_selected_ordering", "get_selected_errors", "get_final_optimal_model_k", "get_loss_all",
     "snn_create", "snn_destroy", "snn_last_error", "snn_select", "snn_select_device",
     "snn_distance_rows", "snn_loss", "snn_set_default_form", "snn_select_batch",
-    "snn_distance_matrix",
+    "snn_distance_matrix", "snn_select_device_noalloc",
 ]
+NOALLOC_SCRATCH_BYTES = 64 << 20  # SNN_NOALLOC_SCRATCH_BYTES
 
 
 class SnnParams(C.Structure):
@@ -87,6 +88,7 @@ def lib() -> C.CDLL:
         L.snn_select_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p,
                                         _i32p, C.c_int, _i32p, C.c_int, C.POINTER(SnnParams),
                                         _i32p, _i32p, _f32p, C.c_void_p, C.POINTER(SnnSummary)]
+        L.snn_select_device_noalloc.argtypes = L.snn_select_device.argtypes
         L.snn_distance_rows.argtypes = [C.c_void_p, _f32p, C.c_int64, C.c_int, C.c_int, _i32p,
                                         C.c_int, C.c_int, _i64p, C.c_int, _f32p, C.c_void_p]
         L.snn_loss.argtypes = [C.c_void_p, _f32p, _f32p, C.c_int64, C.c_int,
@@ -147,8 +149,10 @@ class Engine:
     def select(self, X, y, k_vals, loss_codes, *, method: int, model_type: int, k_fold: int,
                num_features: int, auto_feature: bool = False, form: int = 1,
                candidate_errors: bool = False, device_inputs=None,
-               compensated: bool = False) -> dict:
-        """Run one selection; device_inputs=(X_ptr, y_ptr, n, d) skips the upload."""
+               compensated: bool = False, noalloc: bool = False) -> dict:
+        """Run one selection; device_inputs=(X_ptr, y_ptr, n, d) skips the upload.
+        noalloc (device inputs only): snn_select_device_noalloc, whose device
+        work memory is the library's fixed array."""
         kv = np.ascontiguousarray(k_vals, dtype=np.int32)
         lc = np.ascontiguousarray(loss_codes, dtype=np.int32)
         if device_inputs is None:
@@ -166,13 +170,15 @@ class Engine:
         cand = np.zeros(d * d, np.float32) if candidate_errors else None
         sm = SnnSummary()
         cptr = cand.ctypes.data if cand is not None else None
+        if noalloc and device_inputs is None:
+            raise ValueError("noalloc selections take device inputs")
         if device_inputs is None:
             rc = lib().snn_select(self._ctx, X, n, d, y, kv, len(kv), lc, len(lc), C.byref(p),
                                   sel, order, errs, cptr, C.byref(sm))
         else:
-            rc = lib().snn_select_device(self._ctx, C.c_void_p(xp), n, d, C.c_void_p(yp), kv,
-                                         len(kv), lc, len(lc), C.byref(p), sel, order, errs,
-                                         cptr, C.byref(sm))
+            fn = lib().snn_select_device_noalloc if noalloc else lib().snn_select_device
+            rc = fn(self._ctx, C.c_void_p(xp), n, d, C.c_void_p(yp), kv, len(kv), lc, len(lc),
+                    C.byref(p), sel, order, errs, cptr, C.byref(sm))
         check(rc)
         out = dict(num_final=sm.num_final, optimal_k=sm.optimal_k, steps=sm.steps,
                    selected_dimensions=sel, selected_ordering=order, selected_errors=errs,
@@ -269,6 +275,40 @@ class Engine:
         out = C.c_float()
         check(lib().snn_loss(self._ctx, y, p, len(y), int(code), C.byref(out)))
         return float(out.value)
+
+
+class DeviceMemory:
+    """Minimal libamdhip64 binding for device-resident inputs (snn_select_device)."""
+
+    def __init__(self):
+        self.L = C.CDLL("libamdhip64.so")
+        self.L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.L.hipFree.argtypes = [C.c_void_p]
+        self.L.hipDeviceSynchronize.argtypes = []
+
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        if self.L.hipMalloc(C.byref(p), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed")
+        return p.value
+
+    def upload(self, a: np.ndarray) -> int:
+        a = np.ascontiguousarray(a)
+        p = self.alloc(a.nbytes)
+        if self.L.hipMemcpy(C.c_void_p(p), a.ctypes.data, a.nbytes, 1) != 0:
+            raise RuntimeError("hipMemcpy failed")
+        return p
+
+    def copy_d2d(self, dst: int, src: int, nbytes: int) -> None:
+        if self.L.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), nbytes, 3) != 0:
+            raise RuntimeError("hipMemcpy failed")
+
+    def free(self, p: int) -> None:
+        self.L.hipFree(C.c_void_p(p))
+
+    def sync(self) -> None:
+        self.L.hipDeviceSynchronize()
 
 
 _default_engine = None

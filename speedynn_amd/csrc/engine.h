@@ -86,4 +86,5 @@ struct snn_ctx {
       rank, tauw, iperm, frange, lidx, lval, lcount;
   snn::BatchCache batch;
   EventPool events;
+  void* fixed_scratch = nullptr;  // device address of the fixed work array on this device
 };

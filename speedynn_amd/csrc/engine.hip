@@ -21,6 +21,15 @@
 #include "engine.h"
 #include "kernels.h"
 
+// Device work memory of snn_select_device_noalloc: one fixed array on every
+// GPU the library runs on, held until the process ends.  Its host-side name
+// is only a stand-in: host code uses the device address hipGetSymbolAddress
+// gives for the current device (snn_create keeps it per device), never
+// `scratch` itself.  Calls on one GPU share it, so overlapping calls race;
+// each call initialises every part it reads.
+constexpr size_t kFixedScratchBytes = (size_t)SNN_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) char scratch[kFixedScratchBytes];
+
 using snn::DevBuf;
 using snn::fail;
 using snn::Problem;
@@ -637,6 +646,10 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   return SNN_OK;
 }
 
+// Device address of `scratch` per device ordinal (null: not resolved yet).
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_addr;
+
 std::mutex g_default_mu;
 snn_ctx* g_default = nullptr;
 int g_default_form = SNN_FORM_PRECISION;
@@ -697,8 +710,23 @@ int snn_create(snn_ctx** out, int device) {
     return fail(SNN_EDEVICE, "HIP device " + std::to_string(device) + " not present (" +
                                  std::to_string(count) + " visible)");
   HIPCHK(hipSetDevice(device));
+  void* fixed = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_addr.size() < count) g_scratch_addr.resize(count, nullptr);
+    if (!g_scratch_addr[device]) {
+      hipError_t es = hipGetSymbolAddress(&g_scratch_addr[device], HIP_SYMBOL(scratch));
+      if (es != hipSuccess || !g_scratch_addr[device]) {
+        g_scratch_addr[device] = nullptr;
+        return fail(SNN_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                     hipGetErrorString(es));
+      }
+    }
+    fixed = g_scratch_addr[device];
+  }
   snn_ctx* c = new snn_ctx();
   c->device = device;
+  c->fixed_scratch = fixed;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
@@ -736,6 +764,21 @@ int snn_select_device(snn_ctx* ctx, const float* X_device, int64_t n, int d,
   return select_impl(ctx, X_device, n, d, y_device, k_vals, num_k, loss_functions, num_loss,
                      params, selected_dimensions, selected_ordering, selected_errors,
                      candidate_errors, summary);
+}
+
+int snn_select_device_noalloc(snn_ctx* ctx, const float* X_device, int64_t n, int d,
+                              const float* y_device, const int* k_vals, int num_k,
+                              const int* loss_functions, int num_loss, const snn_params* params,
+                              int* selected_dimensions, int* selected_ordering,
+                              float* selected_errors, float* candidate_errors,
+                              snn_summary* summary) {
+  if (!ctx) return fail(SNN_EINVAL, "null context");
+  if (!X_device || !y_device) return fail(SNN_EINVAL, "null input");
+  if (!ctx->fixed_scratch) return fail(SNN_EDEVICE, "no fixed work array on this device");
+  return select_impl(ctx, X_device, n, d, y_device, k_vals, num_k, loss_functions, num_loss,
+                     params, selected_dimensions, selected_ordering, selected_errors,
+                     candidate_errors, summary, static_cast<char*>(ctx->fixed_scratch),
+                     kFixedScratchBytes);
 }
 
 int snn_select(snn_ctx* ctx, const float* X, int64_t n, int d, const float* y,
