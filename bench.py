@@ -1,25 +1,31 @@
 """Benchmark: greedy kNN wrapper feature selection on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c0|c2|c3|c4]
-                    [--form precision|plain] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c0|c1|c3|c4]
+                    [--form precision|plain] [--no-plain] [--no-cpu-baseline]
     python bench.py --copy-calibration BYTES   # a device copy, for counter scaling
+    python bench.py --gram c2|c4               # the full-d distance matrix alone
 
 One bench step = one complete selection over the workload (every selection
 step: candidate gather, kNN search over all candidates with leave-one-out,
 losses, arg-min, distance update), with X and y already resident in HBM.  The
-default workload is forward selection on n=20000 points, d=200 float32
-features, k=10, 3 classes, to 20 features.  c2: n=65536, d=512, k=20, to 32
-features.  c3: 64 independent problems (n 1000..16000, d 32..256, k 5/10/20,
-to 8 features each) in one batched call per bench step, beside 64 sequential
-plain-form selections.  c4: backward elimination, n=32768, d=1024, k=15,
-regression, to 16 features.
+default workload is forward selection on n=65536 points, d=512 float32
+features with scales 10^U(-2,2), k=20, 2 classes, to 32 features, in the
+precision form, followed by one plain-form selection of the same problem
+(its wall time and whether the two forms' results are bit-identical).  c0:
+n=2000, d=50, k=10 to 10 features.  c1: n=20000, d=200, k=10, 3 classes, to
+20 features.  c3: 64 independent problems (n 1000..16000, d 32..256, k
+5/10/20, to 8 features each) in one batched call per bench step, beside 64
+sequential plain-form selections.  c4: backward elimination, n=32768,
+d=1024, k=15, regression, to 16 features.
 
 Printed: one JSON line with the whole-job throughput in (query x candidate)
 kNN evaluations per second, the selection wall time, the dominant search
-kernel's roofline (algorithmic bytes 4n per evaluation -- the query's D row --
-over the kernel's time from HIP events on the engine's stream; HBM traffic
-from the committed rocprofv3 counter summary when one exists) and the CPU
-oracle's rate on a bounded sample of the same workload.
+kernel's roofline (algorithmic bytes: each query's D row once plus 4 bytes
+per (point, candidate) value the search evaluates, over the kernel's time
+from HIP events on the engine's stream; HBM traffic from the committed
+rocprofv3 counter summary when one exists) and the CPU baselines: the oracle
+on a bounded sample of the same workload, and the reference's own compiled
+CPU module beside the oracle on the n=2000 workload.
 """
 from __future__ import annotations
 
@@ -28,6 +34,7 @@ import ctypes as C
 import json
 import os
 import re
+import subprocess
 import sys
 import time
 
@@ -46,15 +53,18 @@ def list_size(kmax: int) -> int:
 
 
 def traffic_per_launch(workload: str):
-    """HBM bytes per search launch from profiles/traffic_search_<workload>.md, or
-    None.  The summary averages over the pruned-search kernels the autotuner ran
-    in the counted selections (which of them it keeps is decided by timing, so
-    the summary is keyed by workload and names its kernels)."""
+    """(HBM bytes per search launch, source) from the committed rocprofv3
+    counter summary profiles/traffic_search_<workload>.md, or (None, None).
+    The summary averages over the pruned-search launches of the counted
+    selection and names the commit its kernels were built from."""
     path = os.path.join(ROOT, "profiles", f"traffic_search_{workload}.md")
     if not os.path.exists(path):
-        return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", open(path).read())
-    return float(m.group(1)) if m else None
+        return None, None
+    text = open(path).read()
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", text)
+    c = re.search(r"measured_on:\s*(\S+)", text)
+    src = f"profiles/traffic_search_{workload}.md" + (f" (kernels of {c.group(1)})" if c else "")
+    return (float(m.group(1)), src) if m else (None, None)
 
 
 def cpu_baseline(X, y, kv, k_fold, model_type, method, target_s: float = 12.0) -> dict:
@@ -105,6 +115,53 @@ def roofline(alg_bytes: float, ms: float, launches: int, kernel: str, traffic, u
     return out
 
 
+def reference_vs_port_c0(timeout_s: float = 240.0):
+    """The reference's own compiled CPU module (oracle/_ref, built from the
+    reference sources; single-threaded by construction) and the oracle on one
+    thread, each running the whole n=2000 d=50 forward selection, timed in a
+    child process (the reference prints to stdout on some paths).  None where
+    the reference module is not built."""
+    from oracle import oracle as O
+    if not O.ref_available():
+        return None
+    code = (
+        "import json, time, sys\n"
+        f"sys.path.insert(0, {ROOT!r})\n"
+        "from oracle import oracle as O\n"
+        "from speedynn_amd import datasets as ds\n"
+        "w = ds.config('c0')\n"
+        "args = (w['X'], w['y'], w['k_vals'], w['num_features'], w['k_fold'], "
+        "w['loss_functions'], False, w['model_type'], w['method'])\n"
+        "t = time.perf_counter(); r = O.ref_select(*args); tr = time.perf_counter() - t\n"
+        "t = time.perf_counter(); p = O.select(*args, threads=1); tp = time.perf_counter() - t\n"
+        "same = r['selected_ordering'].tolist() == p['selected_ordering'].tolist() and "
+        "r['selected_errors'].tobytes() == p['selected_errors'].tobytes()\n"
+        "print('@@' + json.dumps(dict(reference_s=tr, port_1thread_s=tp, same=same)))\n")
+    try:
+        out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                             timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        return None
+    for line in out.stdout.splitlines():
+        if line.startswith("@@"):
+            return json.loads(line[2:])
+    return None
+
+
+def pruned_kernel_names(pruned_kernel: int, K: int, mult: int, nblk: int) -> str:
+    W = 4 if nblk <= 4 else 8 if nblk <= 8 else 16
+    names = []  # the kernels the autotuner kept for this workload's steps
+    if pruned_kernel & 1:
+        names.append(f"k_search_pruned<{K},{mult}>")
+    if pruned_kernel & 2:
+        names.append(f"k_search_pruned_shared<{K},{mult},{W}>")
+    if pruned_kernel & 4:  # the same kernel, two workgroups per query
+        names.append(f"k_search_pruned_shared<{K},{mult},{4 if nblk <= 8 else 8}> x2")
+    if pruned_kernel & 8:  # the second step's walk along a sorted column
+        names.append(f"k_search_walk<{K}>")
+    return " + ".join(names) or f"k_search_pruned<{K},{mult}>"
+
+
 def bench_single(args, hip, eng, form: int) -> dict:
     from speedynn_amd import datasets as ds
     w = ds.config(args.workload)
@@ -115,10 +172,10 @@ def bench_single(args, hip, eng, form: int) -> dict:
     loss = 4 if mt == 1 else 0
     xp, yp = hip.upload(np.ascontiguousarray(X)), hip.upload(np.ascontiguousarray(y))
     kw = dict(method=method, model_type=mt, k_fold=w["k_fold"], num_features=w["num_features"],
-              form=form, device_inputs=(xp, yp, n, d))
+              device_inputs=(xp, yp, n, d))
 
-    def once():
-        return eng.select(None, None, w["k_vals"], [loss], **kw)
+    def once(f=form):
+        return eng.select(None, None, w["k_vals"], [loss], form=f, **kw)
 
     cold = None  # the process's first selection: the autotuner times its kernels here
     for i in range(args.warmup):
@@ -135,35 +192,33 @@ def bench_single(args, hip, eng, form: int) -> dict:
     r = results[-1]
     steps_sel = r["steps"]
     evals = sum(x["evals"] for x in results)
+    npf = n // w["k_fold"]
 
-    # dominant kernel: the pruned search (precision form; a forward selection's
-    # first step runs over sorted columns instead), else the dense search
+    # dominant kernel: the pruned search of the steps after a forward
+    # selection's first (that one runs over sorted columns), else the dense
+    # search.  Algorithmic bytes per launch: each query's D row once (4n) plus
+    # 4 bytes per candidate value the search evaluates -- the (point,
+    # candidate) pairs it visits, counted by the kernels; points that
+    # exact pruning skips move nothing.
     K = list_size(max(w["k_vals"]))
+    first = 1 if (form == 1 and method == 1) else 0
+    launches = args.steps * (steps_sel - first)
+    ms_dom = sum(x["ms_search"] - (x["ms_search_first"] if first else 0.0) for x in results)
+    full = args.steps * sum((d - t) * n * (n - npf) for t in range(first, steps_sel))
     if form == 1:
-        mult = 1 if method == 1 else -1
-        nblk = (d + 63) // 64
-        W = 4 if nblk <= 4 else 8 if nblk <= 8 else 16
-        names = []  # the kernels the autotuner kept for this workload's steps
-        if r["pruned_kernel"] & 1:
-            names.append(f"k_search_pruned<{K},{mult}>")
-        if r["pruned_kernel"] & 2:
-            names.append(f"k_search_pruned_shared<{K},{mult},{W}>")
-        if r["pruned_kernel"] & 4:  # the same kernel, two workgroups per query
-            names.append(f"k_search_pruned_shared<{K},{mult},{4 if nblk <= 8 else 8}> x2")
-        if r["pruned_kernel"] & 8:  # the second step's walk along a sorted column
-            names.append(f"k_search_walk<{K}>")
-        kernel = " + ".join(names) or f"k_search_pruned<{K},{mult}>"
-        first = 1 if method == 1 else 0
-        ms_dom = sum(x["ms_search"] - (x["ms_search_first"] if first else 0.0) for x in results)
+        kernel = pruned_kernel_names(r["pruned_kernel"], K, 1 if method == 1 else -1,
+                                     (d + 63) // 64)
+        visited = full - sum(x["pruned_point_evals"] for x in results)
     else:
         kernel = f"k_search_dense<{K}>"
-        first = 0
-        ms_dom = sum(x["ms_search"] for x in results)
-    dom_evals = args.steps * sum((d - t) * n for t in range(first, steps_sel))
-    launches = args.steps * (steps_sel - first)
-    rf = roofline(4.0 * n * dom_evals, ms_dom, launches, kernel,
-                  traffic_per_launch(args.workload) if form == 1 else None,
-                  "4*n bytes (one FP32 D row) per (query x candidate) evaluation")
+        visited = full
+    alg = 4.0 * n * n * launches + 4.0 * visited
+    traffic, traffic_src = traffic_per_launch(args.workload) if form == 1 else (None, None)
+    rf = roofline(alg, ms_dom, launches, kernel, traffic,
+                  "4*n per query (its D row) + 4 B per (point, candidate) value the search "
+                  "evaluates; pruned points move nothing")
+    if traffic_src:
+        rf["traffic_source"] = traffic_src
 
     parity = None
     gold = os.path.join(ROOT, "tests", "golden", f"{args.workload}_oracle.json")
@@ -194,16 +249,40 @@ def bench_single(args, hip, eng, form: int) -> dict:
         "update_ms_per_selection": sum(x["ms_update"] for x in results) / args.steps,
         "tally_ms_per_selection": sum(x["ms_tally"] for x in results) / args.steps,
         "distance_ms_per_selection": sum(x["ms_distance"] for x in results) / args.steps,
-        "pruned_point_fraction": (r["pruned_point_evals"] / (r["evals"] * (n - 1))
-                                  if r["evals"] else 0.0),
+        # share of the (point, candidate) evaluations of the pruned steps that
+        # exact pruning skipped
+        "pruned_point_fraction": (1.0 - visited / full) if (form == 1 and full) else 0.0,
         "roofline": rf,
         "parity_vs_golden": parity,
         # forward: features in the order chosen; backward: the first 32 removed
         "selected": (r["selected_ordering"][:steps_sel].tolist() if method == 1 else
                      r["selected_ordering"][::-1][:min(steps_sel, 32)].tolist()),
     }
+    if form == 1 and not args.no_plain:
+        # the plain FP32 form (dense search every step) once, outside the
+        # timed region: its wall time and whether the precision form's
+        # selection and per-step errors are bit-identical to it
+        hip.sync()
+        t1 = time.perf_counter()
+        pl = once(0)
+        hip.sync()
+        plain = time.perf_counter() - t1
+        line["plain_selection_wall_s"] = plain
+        line["speedup_vs_plain"] = plain / (wall / args.steps)
+        line["precision_equals_plain"] = bool(
+            np.array_equal(pl["selected_ordering"], r["selected_ordering"]) and
+            np.array_equal(pl["selected_errors"].view(np.uint32),
+                           r["selected_errors"].view(np.uint32)))
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(X, y, w["k_vals"], w["k_fold"], mt, method)
+        rc0 = reference_vs_port_c0()
+        if rc0:
+            line["cpu_baseline"]["reference_c0"] = {
+                "reference_s": rc0["reference_s"], "port_1thread_s": rc0["port_1thread_s"],
+                "same_selection": rc0["same"],
+                "note": "whole n=2000 d=50 forward selection to 10 features: the reference's "
+                        "own compiled CPU module (single-threaded) beside the oracle on one "
+                        "thread; the reference cannot run this workload (n > 15000 exits)"}
     hip.free(xp)
     hip.free(yp)
     return line
@@ -237,18 +316,26 @@ def bench_batch(args, hip, eng, form: int) -> dict:
     # phase split from one more call without graphs (not part of the timed value)
     _, split = once(graph=False)
     first = 1 if form == 1 else 0
-    dom_bytes = 0.0  # 4*n_p per (query x candidate) evaluation of the dominant kernel
+    # algorithmic bytes of the dominant kernel's launches: each problem's D
+    # rows once per step (4 n_p^2) plus 4 bytes per (point, candidate) value
+    # the search evaluates (pruned points move nothing)
+    rows, full = 0.0, 0
     for p, g in zip(probs, got):
         n, d = p["device_inputs"][2], p["device_inputs"][3]
-        dom_bytes += sum((d - t) * n * 4.0 * n for t in range(first, g["steps"]))
+        npf = n // p["k_fold"]
+        rows += 4.0 * n * n * (g["steps"] - first)
+        full += sum((d - t) * n * (n - npf) for t in range(first, g["steps"]))
+    visited = full - split["pruned_point_evals"] if form == 1 else full
+    dom_bytes = rows + 4.0 * visited
     ms_dom = split["ms_search"] - (split["ms_search_first"] if first else 0.0)
     sizes = sorted({list_size(max(p["k_vals"])) for p in probs})
     launches = len(sizes) * (max(g["steps"] for g in got) - first)  # one per list size and step
     family = "k_batch_pruned" if form == 1 else "k_batch_dense"
     kernel = f"{family}<K> (K = {', '.join(map(str, sizes))})"
     rf = roofline(dom_bytes, ms_dom, launches, kernel,
-                  traffic_per_launch("c3") if form == 1 else None,
-                  "4*n_p bytes per (query x candidate) evaluation of problem p")
+                  traffic_per_launch("c3")[0] if form == 1 else None,
+                  "4*n_p per query of problem p (its D row) + 4 B per (point, candidate) "
+                  "value the search evaluates; pruned points move nothing")
 
     # the comparison: 64 sequential plain-form selections of the same problems
     seq = None
@@ -314,15 +401,15 @@ def bench_gram(args, hip, eng) -> dict:
     return {"gram": args.gram, "n": n, "d": d,
             "mfma_ms": times["mfma"], "exact_ms": times["exact"],
             "mfma_alg_TFLOPs": flops / t_m / 1e12,
-            "mfma_util": 6 * flops / t_m / 2.5e15,  # six bf16 products per term
             "mfma_executed_TFLOPs": executed / t_m / 1e12,
-            "mfma_pipe_util": executed / t_m / 2.5e15,
+            "mfma_util": executed / t_m / 2.5e15,
             "mfma_write_GBps": 4.0 * n * n / t_m / 1e9,
             "exact_alg_TFLOPs": flops / t_e / 1e12,
-            "note": "mfma_util counts six split products per (pair, feature) of the full "
-                    "matrix against the 2.5 PFLOP/s dense BF16 peak; mfma_pipe_util counts "
-                    "the products executed (one triangle, mirrored); D write is 4n^2 bytes; "
-                    "times include the mean and split kernels"}
+            "note": "mfma_util: the bf16 products the matrix cores execute (six split "
+                    "products per feature pair, one triangle of tiles, diagonal tiles whole, "
+                    "features padded to 16) over the time, against the 2.5 PFLOP/s dense "
+                    "BF16 peak; mfma_alg_TFLOPs: 2n^2d over the time; the D write is 4n^2 "
+                    "bytes; times include the mean and split kernels"}
 
 
 def main() -> int:
@@ -330,10 +417,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--form", default="precision", choices=["precision", "plain"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sequential", action="store_true", help="c3: skip the sequential runs")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the one plain-form selection beside the precision form")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="run device-to-device copies of BYTES (counter calibration) and exit")
     ap.add_argument("--gram", choices=["c1", "c2", "c4"], default=None,

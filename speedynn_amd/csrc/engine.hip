@@ -3,6 +3,7 @@
 // kernels per step on one stream (gather candidates, search, losses, arg-min,
 // update); every step reads the device-resident state, so the host issues all
 // steps without synchronising and reads the results once at the end.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -155,20 +156,74 @@ void destroy_events(snn_ctx* ctx) {
 // that scan split over two workgroups per query -- depends on the data (how
 // much the candidate blocks' bounds differ) and on the step (early steps keep
 // many points, late ones few), not only on the shape.  The first step of a
-// shape (n, d, list size, direction) and step range [2^b, 2^(b+1)) in this
-// process runs every applicable kernel on the same step, timed with HIP
-// events, and the fastest serves the rest of that range.  With nc kernels to
-// try, kernel r serves the queries r, r + nc, r + 2 nc, ... of that step: the
+// shape (n, d, list size, direction) and step range [2^b, 2^(b+1)) runs every
+// applicable kernel on the same step, timed with HIP events, and the fastest
+// serves the rest of that range.  With nc kernels to try, kernel r serves the
+// queries r, r + nc, r + 2 nc, ... of that step (the order of the kernels
+// rotates with the range, so no kernel always pays for cold caches): the
 // parts are disjoint and cover every query once, and every kernel gives the
 // same bits, so the step costs about one run, not nc.  (Below 64 nc points
-// each kernel runs the whole step instead; only the last run counts visits.)  A kernel that took more
-// than twice the winner's time is not tried again on that shape's later
-// ranges (the per-wave kernel, which applies everywhere, always is).
-// SPEEDYNN_TUNING=off keeps the rule by direction; so does a call without
-// events (`tune_ev` null), which runs a kernel already chosen for the key.
+// each kernel runs the whole step instead; only the last run counts visits.)
+// Choices persist: SPEEDYNN_TUNE_FILE names a file of choices read at first
+// use and rewritten after every new choice; without it the file tuned.json
+// beside the library is read (never written), so a process starts with the
+// choices committed there.  SPEEDYNN_TUNING=off keeps the rule by direction;
+// so does a call without events (`tune_ev` null), which runs a kernel already
+// chosen for the key.
 std::mutex g_tune_mu;
-std::map<std::tuple<int64_t, int, int, int, int>, int> g_tuned;
-std::map<std::tuple<int64_t, int, int, int>, int> g_slow;  // bit v: variant v lost by 2x
+using TuneKey = std::tuple<int64_t, int, int, int, int>;  // n, d, list size, mult, range
+std::map<TuneKey, int> g_tuned;
+bool g_tune_loaded = false;
+std::string g_tune_write;  // file rewritten after new choices (empty: none)
+
+std::string library_dir() {
+  Dl_info info{};
+  if (!dladdr(reinterpret_cast<void*>(&library_dir), &info) || !info.dli_fname) return ".";
+  std::string p = info.dli_fname;
+  const size_t k = p.rfind('/');
+  return k == std::string::npos ? "." : p.substr(0, k);
+}
+
+// One choice per line: {"n": N, "d": D, "list": K, "mult": M, "range": B, "kernel": V}.
+void load_tuned_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const char* e = std::getenv("SPEEDYNN_TUNE_FILE");
+  std::string path = e && *e ? std::string(e) : library_dir() + "/tuned.json";
+  if (e && *e) g_tune_write = path;
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) return;
+  char line[512];
+  while (std::fgets(line, sizeof line, f)) {
+    long long n = 0;
+    int d = 0, K = 0, mult = 0, b = 0, v = 0;
+    if (std::sscanf(line, " { \"n\": %lld, \"d\": %d, \"list\": %d, \"mult\": %d, \"range\": %d, "
+                          "\"kernel\": %d",
+                    &n, &d, &K, &mult, &b, &v) == 6 &&
+        v >= 1 && v <= 4)
+      g_tuned[std::make_tuple((int64_t)n, d, K, mult, b)] = v;
+  }
+  std::fclose(f);
+}
+
+void save_tuned_locked() {
+  if (g_tune_write.empty()) return;
+  const std::string tmp = g_tune_write + ".tmp";
+  FILE* f = std::fopen(tmp.c_str(), "w");
+  if (!f) return;
+  std::fprintf(f, "[\n");
+  size_t i = 0;
+  for (const auto& kv : g_tuned) {
+    const TuneKey& k = kv.first;
+    std::fprintf(f, "  {\"n\": %lld, \"d\": %d, \"list\": %d, \"mult\": %d, \"range\": %d, "
+                    "\"kernel\": %d}%s\n",
+                 (long long)std::get<0>(k), std::get<1>(k), std::get<2>(k), std::get<3>(k),
+                 std::get<4>(k), kv.second, ++i < g_tuned.size() ? "," : "");
+  }
+  std::fprintf(f, "]\n");
+  std::fclose(f);
+  std::rename(tmp.c_str(), g_tune_write.c_str());
+}
 
 // Variant 4: the walk along the first chosen feature's sorted column, for a
 // forward selection's second step (rank != null); it has a step range of its own.
@@ -208,25 +263,15 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int
     *used |= 1 << (v - 1);
     return SNN_OK;
   }
-  const auto shape = std::make_tuple(sl.n, d, snn::search_list_size(sl.kmax), sl.mult);
   if (!off && nc > 1) {
     std::lock_guard<std::mutex> lock(g_tune_mu);
+    load_tuned_locked();
     auto it = g_tuned.find(key);
     if (it != g_tuned.end()) {
       variant = it->second;
+      if (variant == 4 && !rank) variant = 1;
     } else if (tune_ev) {
-      auto sw = g_slow.find(shape);
-      const int slow = sw == g_slow.end() ? 0 : sw->second;
-      int kept = 0;
-      for (int i = 0; i < nc; ++i)
-        if (cands[i] == 1 || cands[i] == 4 || !(slow & (1 << cands[i]))) cands[kept++] = cands[i];
-      nc = kept;
-      if (nc > 1) {
-        tune = true;
-      } else {
-        variant = cands[0];
-        g_tuned[key] = variant;
-      }
+      tune = true;
     }
   }
   if (!tune) {
@@ -234,6 +279,12 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int
     const int v = variant ? variant : (sl.mult > 0 || !snn::pruned_shared_applies(sl) ? 1 : 2);
     *used |= 1 << (v - 1);
     return SNN_OK;
+  }
+  // the kernels' order rotates with the range
+  {
+    int rot[4];
+    for (int i = 0; i < nc; ++i) rot[i] = cands[(i + bucket) % nc];
+    for (int i = 0; i < nc; ++i) cands[i] = rot[i];
   }
   // every applicable kernel on its part of this step's queries
   hipEvent_t* ev = tune_ev;
@@ -266,9 +317,7 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     g_tuned[key] = variant;
-    int& slow = g_slow[shape];
-    for (int i = 0; i < nc; ++i)
-      if (t[i] > 2.0 * best) slow |= 1 << cands[i];
+    save_tuned_locked();
   }
   return SNN_OK;
 }

@@ -186,6 +186,34 @@ def regression_sum(seed: int, n: int, d: int, n_informative: int, noise: float =
     return X, y.astype(np.float32), inf
 
 
+def scaled_sign(seed: int, n: int, d: int, n_informative: int):
+    """Features N(0,1) times scales 10^U(-2,2), labels as classification_sign
+    on the standardised informative columns (the scales do not change them)."""
+    scales = exp2(_LOG2_10 * (4.0 * uniforms(seed, 5, d) - 2.0))
+    Z = normals(seed, 0, n * d).reshape(n, d)
+    X = (Z * scales[None, :]).astype(np.float32)
+    del Z
+    X, y, inf = classification_sign(seed, n, d, n_informative, 0.05, X=X, standardize=scales)
+    return X, y, inf, scales
+
+
+def correlated_regression(seed: int, n: int, d_base: int, n_informative: int):
+    """d_base N(0,1) columns followed by d_base copies of randomly chosen base
+    columns plus N(0, 0.1^2) noise; y = sum of n_informative base columns plus
+    N(0, 0.5^2)."""
+    base = gaussian_matrix(seed, n, d_base)
+    src = randint(seed, 7, d_base, 0, d_base)
+    noise = normals(seed, 8, n * d_base).reshape(n, d_base)
+    copies = (base[:, src].astype(np.float64) + 0.1 * noise).astype(np.float32)
+    X = np.ascontiguousarray(np.concatenate([base, copies], axis=1))
+    inf = choice(seed, 1, d_base, n_informative)
+    y = np.zeros(n, dtype=np.float64)
+    for c in inf:
+        y = y + X[:, c].astype(np.float64)
+    y = (y + 0.5 * normals(seed, 6, n)).astype(np.float32)
+    return X, y, inf
+
+
 def quantized(X: np.ndarray, step: float) -> np.ndarray:
     """Round features to a grid: produces many exactly tied distances."""
     return (np.round(X.astype(np.float64) / step) * step).astype(np.float32)
@@ -208,31 +236,35 @@ def config(name: str) -> dict:
                     num_features=20, method="forward", model_type="classification",
                     loss_functions=["zero_one"])
     if name == "c2":
-        n, d, seed = 65536, 512, 2
-        scales = exp2(_LOG2_10 * (4.0 * uniforms(seed, 5, d) - 2.0))
-        Z = normals(seed, 0, n * d).reshape(n, d)
-        X = (Z * scales[None, :]).astype(np.float32)
-        del Z
-        X, y, inf = classification_sign(seed, n, d, 32, 0.05, X=X, standardize=scales)
+        X, y, inf, scales = scaled_sign(2, 65536, 512, 32)
         return dict(name=name, X=X, y=y, informative=inf, scales=scales, k_vals=[20],
-                    k_fold=n, num_features=32, method="forward",
+                    k_fold=65536, num_features=32, method="forward",
                     model_type="classification", loss_functions=["zero_one"])
     if name == "c4":
-        n, d, seed = 32768, 1024, 4
-        base = gaussian_matrix(seed, n, 512)
-        src = randint(seed, 7, 512, 0, 512)
-        noise = normals(seed, 8, n * 512).reshape(n, 512)
-        copies = (base[:, src].astype(np.float64) + 0.1 * noise).astype(np.float32)
-        X = np.ascontiguousarray(np.concatenate([base, copies], axis=1))
-        inf = choice(seed, 1, 512, 16)
-        y = np.zeros(n, dtype=np.float64)
-        for c in inf:
-            y = y + X[:, c].astype(np.float64)
-        y = (y + 0.5 * normals(seed, 6, n)).astype(np.float32)
-        return dict(name=name, X=X, y=y, informative=inf, k_vals=[15], k_fold=n,
+        X, y, inf = correlated_regression(4, 32768, 512, 16)
+        return dict(name=name, X=X, y=y, informative=inf, k_vals=[15], k_fold=32768,
                     num_features=16, method="backward", model_type="regression",
                     loss_functions=["rmse"])
     raise ValueError(f"unknown workload {name!r}")
+
+
+def reduced(name: str, n: int, d: int, num_features: int) -> dict:
+    """A workload's recipe at another size: c2's log-scaled sign labels (d
+    features, min(32, d) informative, forward to num_features) or c4's
+    correlated copies (d/2 base columns and d/2 noisy copies, regression on
+    min(16, d/2) base columns, backward to num_features); seeds as the full
+    workload's, so only the sizes differ."""
+    if name == "c2":
+        X, y, inf, scales = scaled_sign(2, n, d, min(32, d))
+        return dict(name=f"c2@{n}x{d}", X=X, y=y, informative=inf, scales=scales,
+                    k_vals=[20], k_fold=n, num_features=num_features, method="forward",
+                    model_type="classification", loss_functions=["zero_one"])
+    if name == "c4":
+        X, y, inf = correlated_regression(4, n, d // 2, min(16, d // 2))
+        return dict(name=f"c4@{n}x{d}", X=X, y=y, informative=inf, k_vals=[15], k_fold=n,
+                    num_features=num_features, method="backward", model_type="regression",
+                    loss_functions=["rmse"])
+    raise ValueError(f"no reduced recipe for {name!r}")
 
 
 def batch_problem_shapes(count: int = 64, seed: int = 1000):

@@ -3,6 +3,7 @@ import glob
 import json
 import os
 import sys
+import zlib
 
 import numpy as np
 import pytest
@@ -31,6 +32,9 @@ def load_case(path):
     elif inp.get("recipe") == "config":
         w = ds.config(inp["config"])
         X, y = w["X"], w["y"]
+    elif inp.get("recipe") == "reduced":
+        w = ds.reduced(inp["config"], inp["n"], inp["d"], inp["num_features"])
+        X, y = w["X"], w["y"]
     elif inp.get("recipe") == "batch_problem":
         w = ds.batch_problem(inp["problem"])
         X, y = w["X"], w["y"]
@@ -48,6 +52,18 @@ def golden_paths(include_large=False):
     if not include_large:
         paths = [p for p in paths if not p.endswith("_oracle.json")]
     return paths
+
+
+def candidate_error_bits(rec):
+    """Every candidate's error at every step of an oracle fixture, as uint32
+    bits [steps][d] (stored as a list or as zlib-compressed base64 of little-endian words)."""
+    e, d = rec["expected"], rec["inputs"]["d"]
+    if "candidate_errors_zb64" in e:
+        raw = zlib.decompress(base64.b64decode(e["candidate_errors_zb64"]))
+        a = np.frombuffer(raw, "<u4").astype(np.uint32)
+    else:
+        a = np.array(e["candidate_errors_bits"], np.uint32)
+    return a.reshape(-1, d)
 
 
 def expected_arrays(rec):

@@ -9,7 +9,8 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, assert_same_result, expected_arrays, golden_paths, load_case
+from conftest import (GOLDEN, assert_same_result, candidate_error_bits, expected_arrays,
+                      golden_paths, load_case)
 from oracle import oracle as O
 from speedynn_amd import _lib, datasets as ds
 
@@ -213,6 +214,28 @@ def test_invalid_arguments_return_errors(engine):
     assert r["num_final"] == 2
 
 
+REDUCED = sorted(os.path.join(GOLDEN, f) for f in os.listdir(GOLDEN)
+                 if f.endswith("_oracle.json") and "_n" in f)
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["plain", "precision"])
+@pytest.mark.parametrize("path", REDUCED, ids=os.path.basename)
+def test_workload_recipes_at_reduced_size_match_oracle(engine, path, form):
+    """The n=65536 forward workload's recipe (log-uniform feature scales, 32
+    informative features, k = 20: two candidate blocks' worth of slots plus a
+    partly filled last block) at n = 3000 with all 512 features, and the
+    n=32768 backward regression recipe (noisy column copies) at n = 1000,
+    d = 320, down to 16: selections, per-step errors and every candidate's
+    error at every step against the oracle."""
+    rec, X, y = load_case(path)
+    print(f"{rec['name']} form={form}", flush=True)
+    got = run(engine, X, y, rec["inputs"], form, candidate_errors=True)
+    assert_same_result(got, expected_arrays(rec), rec["name"])
+    steps = got["steps"]
+    np.testing.assert_array_equal(got["candidate_errors"][:steps].view(np.uint32),
+                                  candidate_error_bits(rec))
+
+
 C1 = os.path.join(GOLDEN, "c1_oracle.json")
 
 
@@ -278,3 +301,35 @@ def test_mfma_gram_within_1e5_of_fp64(engine, n, d):
                 assert got[r][q2] == got[r2][q], (name, q, q2)
         print(f"mfma gram {name}: max relative error {worst:.3e}", flush=True)
         assert worst <= 1e-5, (name, worst)
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(600)
+def test_c2_full_size_precision_equals_plain(engine):
+    """n=65536, d=512, k=20, forward to 32 (beyond the oracle's reach at this
+    size; its recipe is checked against the oracle at n=3000 above): the
+    precision form (sorted first step, warm bounds, row lists, exact pruning)
+    and the plain form (dense search, the reference's operation order) give
+    the same selected sequence, per-step errors and every candidate's error
+    at every step, bit for bit."""
+    w = ds.config("c2")
+    X, y = w["X"], w["y"]
+    n, d = X.shape
+    mem = _lib.DeviceMemory()
+    xp, yp = mem.upload(X), mem.upload(y)
+    try:
+        got = {}
+        for form in (1, 0):
+            print(f"c2 full size form={form}", flush=True)
+            got[form] = engine.select(None, None, w["k_vals"], [4], method=1, model_type=1,
+                                      k_fold=n, num_features=w["num_features"], form=form,
+                                      candidate_errors=True, device_inputs=(xp, yp, n, d))
+    finally:
+        mem.sync()
+        mem.free(xp)
+        mem.free(yp)
+    assert_same_result(got[1], got[0], "c2")
+    steps = got[0]["steps"]
+    assert steps == 32
+    np.testing.assert_array_equal(got[1]["candidate_errors"][:steps].view(np.uint32),
+                                  got[0]["candidate_errors"][:steps].view(np.uint32))

@@ -5,10 +5,13 @@ when the autotuner picks it.  Shapes are chosen so that each kernel applies:
 more than four candidate blocks for the split scan, a second forward step with
 finite X for the walk, and a long backward run in which the bound matrix M is
 recomputed over the remaining candidates (at d/2 and d/4)."""
+import os
+
 import numpy as np
 import pytest
 
-from conftest import assert_same_result
+from conftest import (GOLDEN, assert_same_result, candidate_error_bits, expected_arrays,
+                      load_case)
 from oracle import oracle as O
 from speedynn_amd import _lib, datasets as ds
 
@@ -90,3 +93,30 @@ def test_autotuned_selection_matches_oracle(engine, monkeypatch, case):
     np.testing.assert_array_equal(g["candidate_errors"][:steps].view(np.uint32),
                                   o["candidate_errors"][:steps].view(np.uint32))
     assert bin(g["pruned_kernel"]).count("1") >= 2, g["pruned_kernel"]  # tuning ran
+
+
+REDUCED = [os.path.join(GOLDEN, f) for f in ("c2_n3000_d512_oracle.json",
+                                             "c4_n1000_d320_oracle.json")]
+
+
+@pytest.mark.parametrize("kernel", ["wave", "shared", "split", "walk"])
+@pytest.mark.parametrize("path", REDUCED, ids=lambda p: os.path.basename(p)[:13])
+def test_forced_kernel_on_workload_recipes_matches_oracle(engine, monkeypatch, path, kernel):
+    """Each pruned kernel forced on the n=65536 forward recipe (feature scales
+    10^U(-2,2), 32 informative features, k = 20 over eight candidate blocks,
+    the last partly filled) and on the n=32768 backward recipe (half the
+    columns noisy copies of others, regression, k = 15), at sizes the oracle
+    finishes in a minute: every candidate's error at every step."""
+    rec, X, y = load_case(path)
+    inp = rec["inputs"]
+    print(f"kernel={kernel} {rec['name']}", flush=True)
+    monkeypatch.setenv("SPEEDYNN_PRUNED", kernel)
+    g = engine.select(X, y, inp["k_vals"], [LOSS[inp["loss_functions"][0]]],
+                      method=1 if inp["method"] == "forward" else 0,
+                      model_type=1 if inp["model_type"] == "classification" else 0,
+                      k_fold=inp["k_fold"], num_features=inp["num_features"], form=1,
+                      candidate_errors=True)
+    assert_same_result(g, expected_arrays(rec), rec["name"])
+    steps = g["steps"]
+    np.testing.assert_array_equal(g["candidate_errors"][:steps].view(np.uint32),
+                                  candidate_error_bits(rec))

@@ -180,5 +180,6 @@ def test_bench_reads_committed_traffic_summaries():
     bench = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(bench)
     for w in ("c1", "c2", "c3", "c4"):
-        t = bench.traffic_per_launch(w)
+        t, src = bench.traffic_per_launch(w)
         assert t is not None and t > 0, w
+        assert src.startswith(f"profiles/traffic_search_{w}.md"), src
