@@ -48,14 +48,17 @@ struct Problem {
   int num_desired;
 };
 
-// Per-row lists of the points a forward pruned search may visit, written by
-// the distance update that precedes the search: for row q the entries
+// Per-row lists of the points a pruned search may visit, written by the
+// distance update that precedes the search: for row q the entries
 // e < count[q] of idx/val[q * cap + e] are the eligible points j (outside q's
-// fold), ascending, with D_qj <= the row's largest warm bound; count[q] = -1:
-// no list (no finite warm bound, or more than cap points), scan the row.
+// fold), ascending, whose lower bound (forward D_qj; backward fl(D_qj - M_qj),
+// kept in lb) is at most the row's largest warm bound; val holds D_qj.
+// count[q] = -1: no list (no finite warm bound, or more than cap points),
+// scan the row.
 struct RowList {
   int* idx;
   float* val;
+  float* lb;  // backward only (null forward: the lower bound is val)
   int* count;
   int64_t cap;
 };
@@ -98,7 +101,7 @@ struct SearchProblem {
   const float* XT;
   const float* frange;
   int d;
-  RowList rows;  // forward: short lists instead of D rows where count >= 0 (idx null: none)
+  RowList rows;  // short lists instead of D (and M) rows where count >= 0 (idx null: none)
 };
 
 // Queries a pruned-search launch covers, and the point of the i-th.

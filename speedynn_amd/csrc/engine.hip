@@ -332,7 +332,7 @@ struct Work {
   double* key = nullptr;
   float *M = nullptr, *skey = nullptr, *med = nullptr, *tauw = nullptr, *frange = nullptr;
   int *lidx = nullptr, *lcount = nullptr;
-  float* lval = nullptr;
+  float *lval = nullptr, *llb = nullptr;
   int *seeds = nullptr, *sidx = nullptr, *rank = nullptr;
   unsigned long long* visited = nullptr;
   void* sort_tmp = nullptr;
@@ -475,13 +475,14 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     bd.add(ctx->iperm, sizeof(int) * d, &w.iperm);
     bd.add(ctx->frange, sizeof(float) * 2, &w.frange);
   }
-  // forward precision form: the update writes each row's list of points the
+  // precision form: the fused update writes each row's list of points the
   // next search may visit (up to n/8 entries, at least 256, per row)
   const int64_t rows_cap =
-      (pruned && fwd && n % 4 == 0) ? std::min<int64_t>(n, std::max<int64_t>(256, n / 8)) : 0;
+      (pruned && n % 4 == 0) ? std::min<int64_t>(n, std::max<int64_t>(256, n / 8)) : 0;
   if (rows_cap) {
     bd.add(ctx->lidx, sizeof(int) * (size_t)n * rows_cap, &w.lidx);
     bd.add(ctx->lval, sizeof(float) * (size_t)n * rows_cap, &w.lval);
+    if (!fwd) bd.add(ctx->llb, sizeof(float) * (size_t)n * rows_cap, &w.llb);
     bd.add(ctx->lcount, sizeof(int) * (size_t)n, &w.lcount);
   }
   if (sorted_first) {
@@ -565,7 +566,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     sl.iperm = w.iperm;
     sl.XT = w.XT;
     sl.frange = w.frange;
-    if (rows_cap) sl.rows = snn::RowList{w.lidx, w.lval, w.lcount, rows_cap};
+    if (rows_cap) sl.rows = snn::RowList{w.lidx, w.lval, w.llb, w.lcount, rows_cap};
   }
 
   // per step: [a] gather [b] bound seeds [b2] search [c] losses + arg-min
@@ -635,7 +636,8 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     if (step + 1 < pb.max_steps) {
       if (fused_seeds)
         HIPCHK(snn::launch_update_seeds(D, Dlo, w.XT, n, st, fwd ? 1 : -1, w.seeds, pb.npf,
-                                        sl.rows, w.tauw, w.frange, d, w.cand, w.iperm, s));
+                                        sl.rows, w.tauw, w.frange, d, w.cand, w.iperm, w.M,
+                                        s));
       else
         HIPCHK(snn::launch_update(D, Dlo, w.XT, n, st, -1, fwd ? 1 : -1, s));
     }
@@ -794,7 +796,7 @@ void snn_destroy(snn_ctx* c) {
                     &c->kvals, &c->losses, &c->state, &c->scratch, &c->seeds,
                     &c->visited, &c->perm, &c->key, &c->M, &c->med, &c->skey,
                     &c->sidx, &c->sort_tmp, &c->rank, &c->tauw, &c->iperm, &c->frange,
-                    &c->lidx, &c->lval, &c->lcount};
+                    &c->lidx, &c->lval, &c->llb, &c->lcount};
   for (DevBuf* b : bufs) b->release();
   destroy_events(c);
   c->batch.drop_graphs();

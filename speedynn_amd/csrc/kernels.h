@@ -16,12 +16,14 @@ hipError_t launch_max_square(const float* Xc, const SelState* st, float* M, int6
                              hipStream_t s);
 // The update of the device state's pending feature fused with the seed scan
 // of the next step (seeds as launch_seed_scan writes them); n % 4 == 0.
-// Forward: rl.idx set also writes the next search's row lists (RowList) from
-// the warm bounds tauw and the chosen column's range frange.
+// rl.idx set also writes the next search's row lists (RowList) from the warm
+// bounds tauw (forward: and the chosen column's range frange; backward: the
+// lower bounds fl(D - M) from M).
 hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
                                const SelState* st, int mult, int* seeds, int64_t npf,
                                const RowList& rl, const float* tauw, const float* frange, int d,
-                               const int* cand, const int* iperm, hipStream_t s);
+                               const int* cand, const int* iperm, const float* M,
+                               hipStream_t s);
 hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const SelState* st,
                          int f, int mult, hipStream_t s);
 hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* Xc, int64_t n,

@@ -303,13 +303,16 @@ __global__ __launch_bounds__(256) void k_update(float* __restrict__ D, float* __
 // L is offered the points 4L .. 4L + 3 of every 256); the row's 128 seeds are
 // then written ranked by D, as k_seed_scan writes them.  Saves the seed
 // scan's full read of D on every pruned step.
-template <bool COMP>
 //
-// Forward (rl.idx set): the row's list for the next search (RowList) is
-// written as well.  Its bound is the largest warm bound over the next step's
-// candidates (cand[c < ncand], k_select has listed them): fl(max tau + Q)
-// widened as in the search's warm bound; no list where some candidate's tau
-// is unknown (kMaxF) or X is not finite.
+// With rl.idx set the row's list for the next search (RowList) is written as
+// well.  Its bound is the largest warm bound over the next step's candidates
+// (cand[c < ncand], k_select has listed them): forward fl(max tau + Q)
+// widened as in the search's warm bound, backward max tau itself; no list
+// where some candidate's tau is unknown (kMaxF) or X is not finite.  The
+// backward lower bound fl(D - M) uses the M rows as they stand; a later
+// refresh only raises it, so the list still holds every point the search
+// can visit.
+template <bool COMP>
 __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
                                                       float* __restrict__ Dlo,
                                                       const float* __restrict__ XT, int64_t n,
@@ -318,7 +321,8 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
                                                       RowList rl, const float* __restrict__ tauw,
                                                       const float* __restrict__ frange, int d,
                                                       const int* __restrict__ cand,
-                                                      const int* __restrict__ iperm) {
+                                                      const int* __restrict__ iperm,
+                                                      const float* __restrict__ M) {
   if (!st->pending) return;
   const int f = st->fstar;
   const int lane = threadIdx.x & 63;
@@ -347,16 +351,24 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
       if (!__any(unknown)) {
-        const float lo = xi - frange[0], hi = frange[1] - xi;
-        const float w = fmaf(m + fmaxf(lo * lo, hi * hi), 1.00000095367431640625f, 1.0e-36f);
-        if (w < kInf) Tq = w;
+        if (mult < 0) {
+          Tq = m;
+        } else {
+          const float lo = xi - frange[0], hi = frange[1] - xi;
+          const float w = fmaf(m + fmaxf(lo * lo, hi * hi), 1.00000095367431640625f, 1.0e-36f);
+          if (w < kInf) Tq = w;
+        }
       }
     }
     int64_t lcount = 0;  // list entries written (wave-uniform)
     int* __restrict__ lidx = rl.idx ? rl.idx + i * rl.cap : nullptr;
     float* __restrict__ lval = rl.idx ? rl.val + i * rl.cap : nullptr;
-    // in: the lane holds four points of the row (all lanes call this together)
-    auto upd = [&](v4f& dv, v4f& lv, const v4f xj, int64_t j, bool in) {
+    float* __restrict__ llb = rl.lb ? rl.lb + i * rl.cap : nullptr;
+    const float* __restrict__ mrow = rl.lb ? M + i * n : nullptr;  // backward lists
+    // in: the lane holds four points of the row (all lanes call this together);
+    // mv: their M values (backward lists)
+    v4f lbv = {0.f, 0.f, 0.f, 0.f};
+    auto upd = [&](v4f& dv, v4f& lv, const v4f xj, const v4f mv, int64_t j, bool in) {
       int keep = 0;
       if (in) {
 #pragma unroll
@@ -368,7 +380,8 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
           if (COMP) lv[e] += comp_error(xi, xj[e], t, p, dv[e], ps, s, mult < 0);
           dv[e] = s;
           sp.offer(s, j + e, flo, fhi);
-          keep |= (s <= Tq && (j + e < flo || j + e >= fhi)) ? 1 << e : 0;
+          lbv[e] = mrow ? s - mv[e] : s;
+          keep |= (lbv[e] <= Tq && (j + e < flo || j + e >= fhi)) ? 1 << e : 0;
         }
       }
       if (Tq != -kInf) {
@@ -392,6 +405,7 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
             if (keep & (1 << e)) {
               lidx[pos] = (int)(j + e);
               lval[pos] = dv[e];
+              if (llb) llb[pos] = lbv[e];
               ++pos;
             }
           }
@@ -407,16 +421,18 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
     int64_t jb = 0;
     for (; jb + U * 256 <= n; jb += U * 256) {
       const int64_t j = jb + (int64_t)lane * 4;
-      v4f dv[U], lv[U], xj[U];
+      v4f dv[U], lv[U], xj[U], mv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         xj[u] = *reinterpret_cast<const v4f*>(xf + j + u * 256);
         dv[u] = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j + u * 256));
         if (COMP) lv[u] = *reinterpret_cast<v4f*>(rlo + j + u * 256);
+        mv[u] = mrow ? __builtin_nontemporal_load(reinterpret_cast<const v4f*>(mrow + j + u * 256))
+                     : v4f{0.f, 0.f, 0.f, 0.f};
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        upd(dv[u], lv[u], xj[u], j + u * 256, true);
+        upd(dv[u], lv[u], xj[u], mv[u], j + u * 256, true);
         __builtin_nontemporal_store(dv[u], reinterpret_cast<v4f*>(row + j + u * 256));
         if (COMP) *reinterpret_cast<v4f*>(rlo + j + u * 256) = lv[u];
       }
@@ -425,12 +441,14 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
       const int64_t j = jb + (int64_t)lane * 4;
       const bool in = j < n;
       v4f lv = {0.f, 0.f, 0.f, 0.f}, xj = {0.f, 0.f, 0.f, 0.f}, dv = {0.f, 0.f, 0.f, 0.f};
+      v4f mv = {0.f, 0.f, 0.f, 0.f};
       if (in) {
         xj = *reinterpret_cast<const v4f*>(xf + j);
         dv = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j));
         if (COMP) lv = *reinterpret_cast<v4f*>(rlo + j);
+        if (mrow) mv = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(mrow + j));
       }
-      upd(dv, lv, xj, j, in);
+      upd(dv, lv, xj, mv, j, in);
       if (in) {
         __builtin_nontemporal_store(dv, reinterpret_cast<v4f*>(row + j));
         if (COMP) *reinterpret_cast<v4f*>(rlo + j) = lv;
@@ -1197,16 +1215,17 @@ hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const
 hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
                                const SelState* st, int mult, int* seeds, int64_t npf,
                                const RowList& rl, const float* tauw, const float* frange, int d,
-                               const int* cand, const int* iperm, hipStream_t s) {
+                               const int* cand, const int* iperm, const float* M,
+                               hipStream_t s) {
   if (n % 4) return hipErrorInvalidValue;
   const int64_t rows = (n + 3) / 4;
   const int g = (int)(rows < 8192 ? rows : 8192);
   if (Dlo)
     hipLaunchKernelGGL((k_update_seeds<true>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st, mult,
-                       seeds, npf, rl, tauw, frange, d, cand, iperm);
+                       seeds, npf, rl, tauw, frange, d, cand, iperm, M);
   else
     hipLaunchKernelGGL((k_update_seeds<false>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st, mult,
-                       seeds, npf, rl, tauw, frange, d, cand, iperm);
+                       seeds, npf, rl, tauw, frange, d, cand, iperm, M);
   return hipGetLastError();
 }
 

@@ -10,11 +10,11 @@
 #include "kernels.h"
 
 // Waves per SIMD the forward per-wave pruned search is compiled for (the
-// register budget of its body): lists up to 16 fit six waves (80 registers);
-// the 20-entry list runs five (96 registers: at 80 it spills, which cost the
+// register budget of its body): lists up to 10 fit six waves (80 registers);
+// longer ones run five (96 registers: at 80 they spill, which cost the
 // n = 65536 forward selection 40% of its late-step search time).
 #ifndef SNN_PRUNED_WAVES
-#define SNN_PRUNED_WAVES(K) ((K) <= 16 ? 6 : 5)
+#define SNN_PRUNED_WAVES(K) ((K) <= 10 ? 6 : 5)
 #endif
 
 namespace snn {
@@ -289,6 +289,10 @@ __device__ __forceinline__ float seed_bound(const SearchProblem& a, const float*
                                             const float* __restrict__ Dq, float xq, int64_t q,
                                             int cpad, bool live, float warm) {
   const float kInf = __int_as_float(0x7f800000);
+  // backward: the warm bound is the previous step's K-th distance itself,
+  // which the seeds rarely improve; where every live lane has one, the 32
+  // seed gathers are skipped
+  if (MULT < 0 && !__ballot(live && !(warm < kInf))) return live ? warm : -kInf;
   float B[K];
 #pragma unroll
   for (int m = 0; m < K; ++m) B[m] = kInf;
@@ -405,10 +409,17 @@ constexpr int pend_slots() {
 }
 
 // Words of LDS one wave of the per-wave pruned kernels uses: the pending
-// buffer during the scan, the epilogue labels (K slots x 64 lanes) after it.
+// buffer during the scan, the epilogue labels (K slots x 64 lanes) after it,
+// and then the ring of visited points (kRing (index, D) pairs, from word
+// ring_offset<K>()).
+constexpr int kRing = 128;
+template <int K>
+constexpr int ring_offset() {
+  return 64 * (K > 2 * pend_slots<K>() ? K : 2 * pend_slots<K>());
+}
 template <int K>
 constexpr int wave_lds_words() {
-  return 64 * (K > 2 * pend_slots<K>() ? K : 2 * pend_slots<K>());
+  return ring_offset<K>() + 2 * kRing;
 }
 
 // Flush of the pending buffer `buf` (the lane's slot 0; `pc` entries): insert
@@ -528,80 +539,127 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
     return MULT < 0 ? __builtin_nontemporal_load(Mq + (j < n32 ? j : n32 - 1)) : 0.0f;
   };
   const uint32_t ucpad = (uint32_t)cpad;  // n * cpad < 2^31 (checked on the host)
-  // the visits of one chunk: the points of `mask`, ascending; jof(b) is the
-  // point index of lane b
-  auto visit = [&](auto jof, float dcur, unsigned long long mask) {
-    while (mask) {
-      constexpr int BATCH = 8;
-      int js[BATCH];
-      float dqs[BATCH], ts[BATCH];
-      bool pass[BATCH];
-      int cnt = 0;
+  // Visits.  Each chunk's points to visit (the set bits of its ballot) are
+  // appended, in ascending point order, to a ring of (index, D) pairs in the
+  // wave's LDS area by one lane-parallel store, and evaluated eight at a time
+  // from the ring's head: four broadcast 16-byte LDS reads, the gathers of the
+  // 64 candidates' values (256 B each, coalesced) and the screen, with the
+  // batch's bookkeeping in a few scalar instructions, and batches that stay
+  // full across chunk boundaries.  At most 7 + 64 entries are pending, so the
+  // ring (128) never overflows and full batches start at multiples of 8.  The
+  // order of evaluation is the order of the points, so the pending buffer
+  // still receives ascending indices.
+  int2* __restrict__ ring = reinterpret_cast<int2*>(buf - lane + ring_offset<K>());
+  int rhead = 0, rtail = 0;  // wave-uniform: entries [rhead, rtail) are pending
+  constexpr int BATCH = 8;
+  // screen of a batch (cnt valid entries); the exact test runs only if some
+  // lane may pass
+  auto screen = [&](const int (&js)[BATCH], const float (&dqs)[BATCH], float (&ts)[BATCH],
+                    int cnt) {
+    bool pass[BATCH];
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < BATCH; ++u) {
+      ts[u] = xq - ts[u];
+      const float scr = MULT > 0 ? fmaf(ts[u], ts[u], dqs[u]) : dist(dqs[u], ts[u] * ts[u]);
+      pass[u] = u < cnt && scr < lim;
+      any |= pass[u];
+    }
+    if (__ballot(any)) {
 #pragma unroll
       for (int u = 0; u < BATCH; ++u) {
-        if (mask) {
-          const int b = __ffsll((long long)mask) - 1;
-          mask &= mask - 1;
-          js[u] = jof(b);
-          dqs[u] = lane_value(dcur, b);
-          ts[u] = Xl[(uint32_t)js[u] * ucpad];
-          cnt = u + 1;
-        }
-      }
-      // screen the batch; the exact test runs only if some lane may pass
-      bool any = false;
-#pragma unroll
-      for (int u = 0; u < BATCH; ++u) {
-        ts[u] = xq - ts[u];
-        const float scr = MULT > 0 ? fmaf(ts[u], ts[u], dqs[u]) : dist(dqs[u], ts[u] * ts[u]);
-        pass[u] = u < cnt && scr < lim;
-        any |= pass[u];
-      }
-      if (__ballot(any)) {
-#pragma unroll
-        for (int u = 0; u < BATCH; ++u) {
-          if (pass[u]) {
-            const float dd = dist(dqs[u], ts[u] * ts[u]);
-            if (dd < Ld[K - 1]) {
-              buf[pc * 64] = dd;
-              buf[(pend_slots<K>() + pc) * 64] = __int_as_float(js[u]);
-              ++pc;
-            }
+        if (pass[u]) {
+          const float dd = dist(dqs[u], ts[u] * ts[u]);
+          if (dd < Ld[K - 1]) {
+            buf[pc * 64] = dd;
+            buf[(pend_slots<K>() + pc) * 64] = __int_as_float(js[u]);
+            ++pc;
           }
         }
-        if (__ballot(pc > pend_slots<K>() - BATCH)) flush();
       }
+      if (__ballot(pc > pend_slots<K>() - BATCH)) flush();
     }
   };
-  const int lcnt = (MULT > 0 && a.rows.idx) ? a.rows.count[q] : -1;
+  auto eval8 = [&]() {  // the next eight entries (rhead % 8 == 0)
+    const int4* r4 = reinterpret_cast<const int4*>(ring + (rhead & (kRing - 1)));
+    int4 e[4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) e[h] = r4[h];  // broadcast reads, all issued first
+    int js[BATCH];
+    float dqs[BATCH], ts[BATCH];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      js[2 * h] = __builtin_amdgcn_readfirstlane(e[h].x);
+      dqs[2 * h] = __int_as_float(__builtin_amdgcn_readfirstlane(e[h].y));
+      js[2 * h + 1] = __builtin_amdgcn_readfirstlane(e[h].z);
+      dqs[2 * h + 1] = __int_as_float(__builtin_amdgcn_readfirstlane(e[h].w));
+    }
+#pragma unroll
+    for (int u = 0; u < BATCH; ++u) ts[u] = Xl[(uint32_t)js[u] * ucpad];
+    rhead += BATCH;
+    screen(js, dqs, ts, BATCH);
+  };
+  auto eval_tail = [&]() {  // the last (< 8) entries
+    const int cnt = rtail - rhead;
+    int js[BATCH];
+    float dqs[BATCH], ts[BATCH];
+#pragma unroll
+    for (int u = 0; u < BATCH; ++u) {
+      const int2 e = ring[(rhead + (u < cnt ? u : 0)) & (kRing - 1)];
+      js[u] = __builtin_amdgcn_readfirstlane(e.x);
+      dqs[u] = __int_as_float(__builtin_amdgcn_readfirstlane(e.y));
+      ts[u] = u < cnt ? Xl[(uint32_t)js[u] * ucpad] : 0.0f;
+    }
+    rhead = rtail;
+    screen(js, dqs, ts, cnt);
+  };
+  // a chunk's points to visit: lane l holds point jv with D value dcur, and
+  // bit l of mask says whether to visit it
+  auto visit = [&](int jv, float dcur, unsigned long long mask) {
+    if ((mask >> lane) & 1ull) {
+      const int r = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+      ring[(rtail + r) & (kRing - 1)] = make_int2(jv, __float_as_int(dcur));
+    }
+    __builtin_amdgcn_wave_barrier();  // the stores above precede the reads of other lanes
+    rtail += (int)__popcll(mask);
+    while (rtail - rhead >= BATCH) eval8();
+  };
+  const int lcnt = a.rows.idx ? a.rows.count[q] : -1;
   if (lcnt >= 0) {
     // the row's list (RowList): its entries ascend in point index, lie
-    // outside q's fold and hold every point with D <= T; the same chunked
-    // scan with two chunks of 64 entries in flight
-    const int* __restrict__ li = a.rows.idx + q * a.rows.cap;
-    const float* __restrict__ lv = a.rows.val + q * a.rows.cap;
+    // outside q's fold and hold every point whose lower bound is <= T; the
+    // same chunked scan with two chunks of 64 entries in flight
+    const int64_t lbase = q * a.rows.cap;
+    const int* __restrict__ li = a.rows.idx + lbase;
+    const float* __restrict__ lv = a.rows.val + lbase;
+    const float* __restrict__ ll = MULT > 0 ? lv : a.rows.lb + lbase;
     const int last = lcnt > 0 ? lcnt - 1 : 0;
-    auto ldI = [&](int e) { return __builtin_nontemporal_load(li + (e < lcnt ? e : last)); };
-    auto ldV = [&](int e) { return __builtin_nontemporal_load(lv + (e < lcnt ? e : last)); };
+    // cached loads: every candidate block of the query reads the same list
+    auto ldI = [&](int e) { return li[e < lcnt ? e : last]; };
+    auto ldV = [&](int e) { return lv[e < lcnt ? e : last]; };
+    auto ldL = [&](int e) { return MULT > 0 ? 0.0f : ll[e < lcnt ? e : last]; };
     int jsl[2];
-    float dsl[2];
+    float dsl[2], lsl[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       jsl[u] = ldI(64 * u + lane);
       dsl[u] = ldV(64 * u + lane);
+      lsl[u] = ldL(64 * u + lane);
     }
     for (int g0 = 0; g0 < lcnt; g0 += 128) {
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int e = g0 + 64 * u + lane;
-        const unsigned long long mask = __ballot(e < lcnt && dsl[u] <= T);
+        const float lbe = MULT > 0 ? dsl[u] : lsl[u];
+        const unsigned long long mask = __ballot(e < lcnt && lbe <= T);
         if (mask) {
           seen += __popcll(mask);
-          const int jv = jsl[u];
-          visit([&](int b) { return __builtin_amdgcn_readlane(jv, b); }, dsl[u], mask);
+          visit(jsl[u], dsl[u], mask);
         }
         jsl[u] = ldI(e + 128);
         dsl[u] = ldV(e + 128);
+        lsl[u] = ldL(e + 128);
       }
     }
   } else {
@@ -621,8 +679,7 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
       const unsigned long long mask = __ballot(ok);
       if (mask) {
         seen += __popcll(mask);
-        const int j0 = g0 + 64 * u;
-        visit([=](int b) { return j0 + b; }, dsl[u], mask);
+        visit(jl, dsl[u], mask);
       }
       // the slot's next chunk, issued once its value is dead, so the load
       // lands in the slot's own register
@@ -631,6 +688,7 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
     }
   }
   }
+  if (rtail > rhead) eval_tail();
   flush();
   if (a.visited && lane == 0) {
     const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
