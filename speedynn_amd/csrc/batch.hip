@@ -289,7 +289,7 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     g.pruned_start = take(moff, sizeof(int64_t) * (g.count + 1));
   }
   plan.counter_off = take(moff, sizeof(unsigned long long));
-  plan.visited_off = take(moff, sizeof(unsigned long long));
+  plan.visited_off = take(moff, sizeof(unsigned long long) * snn::kVisitSlots * snn::kVisitStride);
 
   HIPCHK(hipSetDevice(ctx->device));
   snn::BatchCache& cache = ctx->batch;
@@ -365,7 +365,9 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
                           hipMemcpyHostToDevice, s));
   }
   HIPCHK(hipEventRecord(e0, s));
-  if (pruned) HIPCHK(snn::launch_fill_u64(at<unsigned long long>(cache.meta, plan.visited_off), 1, s));
+  if (pruned)
+    HIPCHK(snn::launch_fill_u64(at<unsigned long long>(cache.meta, plan.visited_off),
+                                snn::kVisitSlots * snn::kVisitStride, s));
   for (const Slot& sl : plan.slots) {
     const Problem& pb = sl.pb;
     const int64_t nn = pb.n * pb.n;
@@ -499,8 +501,10 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     summary->evals = evals;
     if (pruned) {
       unsigned long long done = 0;
-      HIPCHK(hipMemcpy(&done, at<unsigned long long>(cache.meta, plan.visited_off), sizeof(done),
-                       hipMemcpyDeviceToHost));
+      std::vector<unsigned long long> slots((size_t)snn::kVisitSlots * snn::kVisitStride);
+      HIPCHK(hipMemcpy(slots.data(), at<unsigned long long>(cache.meta, plan.visited_off),
+                       sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost));
+      for (int i = 0; i < snn::kVisitSlots; ++i) done += slots[(size_t)i * snn::kVisitStride];
       summary->pruned_point_evals = pruned_full - (int64_t)done;
     }
   }

@@ -74,7 +74,8 @@ struct SearchProblem {
   const SelState* st;    // device state
   const int* seeds;      // [n][128] pruning seeds
   float* P;              // [nk][cpad][n] predictions
-  unsigned long long* visited;  // point evaluations performed (pruned search)
+  unsigned long long* visited;  // point evaluations performed (pruned search): kVisitSlots
+                                // counters kVisitStride apart, see count_visits
   int64_t n;
   int64_t npf;           // points per fold
   int cpad;
@@ -103,6 +104,18 @@ struct SearchProblem {
   int d;
   RowList rows;  // short lists instead of D (and M) rows where count >= 0 (idx null: none)
 };
+
+// Counters of the point evaluations a pruned search performs: one counter
+// per wave would be as many same-address atomics as waves (a backward step
+// at n = 32768 launches half a million), which serialise at one L2 channel;
+// waves add to one of kVisitSlots counters, each on its own 128-byte line,
+// and the host sums them.
+constexpr int kVisitSlots = 512;
+constexpr int kVisitStride = 16;  // unsigned long longs per counter: 128 bytes
+__device__ __forceinline__ void count_visits(unsigned long long* visited, unsigned int wave_id,
+                                             unsigned long long v) {
+  atomicAdd(visited + (size_t)(wave_id & (kVisitSlots - 1)) * kVisitStride, v);
+}
 
 // Queries a pruned-search launch covers, and the point of the i-th.
 __host__ __device__ inline int64_t launch_queries(const SearchProblem& a) {

@@ -470,7 +470,8 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   if (pruned && !fwd) bd.add(ctx->M, sizeof(float) * nn, &w.M);
   if (pruned) {
     bd.add(ctx->seeds, sizeof(int) * (size_t)n * 128, &w.seeds);
-    bd.add(ctx->visited, sizeof(unsigned long long), &w.visited);
+    bd.add(ctx->visited, sizeof(unsigned long long) * snn::kVisitSlots * snn::kVisitStride,
+           &w.visited);
     bd.add(ctx->tauw, sizeof(float) * (size_t)n * d, &w.tauw);
     bd.add(ctx->iperm, sizeof(int) * d, &w.iperm);
     bd.add(ctx->frange, sizeof(float) * 2, &w.frange);
@@ -507,7 +508,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
                         s));
   if (w.cand_err) HIPCHK(snn::launch_fill(w.cand_err, (int64_t)d * d, std::nanf(""), s));
   if (pruned) {
-    HIPCHK(snn::launch_fill_u64(w.visited, 1, s));
+    HIPCHK(snn::launch_fill_u64(w.visited, snn::kVisitSlots * snn::kVisitStride, s));
     HIPCHK(snn::launch_fill(w.tauw, (int64_t)n * d, snn::kMaxF, s));  // no warm bound yet
     if (w.lcount) HIPCHK(snn::launch_fill_i32(w.lcount, n, -1, s));   // no row lists yet
   }
@@ -689,7 +690,10 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     summary->evals = ev;
     if (pruned) {
       unsigned long long done = 0;
-      HIPCHK(hipMemcpy(&done, w.visited, sizeof(done), hipMemcpyDeviceToHost));
+      std::vector<unsigned long long> slots((size_t)snn::kVisitSlots * snn::kVisitStride);
+      HIPCHK(hipMemcpy(slots.data(), w.visited, sizeof(unsigned long long) * slots.size(),
+                       hipMemcpyDeviceToHost));
+      for (int i = 0; i < snn::kVisitSlots; ++i) done += slots[(size_t)i * snn::kVisitStride];
       summary->pruned_point_evals = pruned_full - (int64_t)done;
     }
   }

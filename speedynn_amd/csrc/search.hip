@@ -692,7 +692,8 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
   flush();
   if (a.visited && lane == 0) {
     const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
-    atomicAdd(a.visited, seen * (unsigned long long)nlive);  // point evaluations performed
+    count_visits(a.visited, blockIdx.x * 4 + (threadIdx.x >> 6),
+                 seen * (unsigned long long)nlive);  // point evaluations performed
   }
   if (a.tauw && live) a.tauw[q * a.d + rk] = Ld[K - 1];  // the next step's warm bound
   finish_query<K>(a, Li, buf, q, c, live);
@@ -987,7 +988,8 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
   flush();
   if (a.visited && lane == 0 && active) {
     const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
-    atomicAdd(a.visited, seen * (unsigned long long)nlive);
+    count_visits(a.visited, (blockIdx.x + blockIdx.y * gridDim.x) * W + wave,
+                 seen * (unsigned long long)nlive);
   }
   if (a.tauw && live) a.tauw[(int64_t)q * a.d + rk] = Ld[K - 1];
   // epilogue: the labels go to the wave's own pending area (empty after the
@@ -1145,7 +1147,7 @@ __global__ __launch_bounds__(256) void k_search_walk(const SearchProblem a,
   }
   if (a.visited && lane == 0) {
     const int nlive = ncand - cb * 64 < 64 ? ncand - cb * 64 : 64;
-    atomicAdd(a.visited, seen * (unsigned long long)nlive);
+    count_visits(a.visited, blockIdx.x * 4 + wave, seen * (unsigned long long)nlive);
   }
   if (a.tauw && live) a.tauw[q * a.d + rk] = Ld[K - 1];
   finish_query<K>(a, Li, lab, q, c, live);
