@@ -333,6 +333,7 @@ struct Work {
   float *M = nullptr, *skey = nullptr, *med = nullptr, *tauw = nullptr, *frange = nullptr;
   int *lidx = nullptr, *lcount = nullptr;
   float *lval = nullptr, *llb = nullptr;
+  unsigned short* M16 = nullptr;
   int *seeds = nullptr, *sidx = nullptr, *rank = nullptr;
   unsigned long long* visited = nullptr;
   void* sort_tmp = nullptr;
@@ -483,7 +484,10 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   if (rows_cap) {
     bd.add(ctx->lidx, sizeof(int) * (size_t)n * rows_cap, &w.lidx);
     bd.add(ctx->lval, sizeof(float) * (size_t)n * rows_cap, &w.lval);
-    if (!fwd) bd.add(ctx->llb, sizeof(float) * (size_t)n * rows_cap, &w.llb);
+    if (!fwd) {
+      bd.add(ctx->llb, sizeof(float) * (size_t)n * rows_cap, &w.llb);
+      bd.add(ctx->M16, sizeof(unsigned short) * nn, &w.M16);
+    }
     bd.add(ctx->lcount, sizeof(int) * (size_t)n, &w.lcount);
   }
   if (sorted_first) {
@@ -533,6 +537,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     if (Dlo) HIPCHK(snn::launch_fill(Dlo, (int64_t)nn, 0.0f, s));
   } else {
     HIPCHK(snn::launch_gram_exact(w.XT, D, Dlo, pruned ? w.M : nullptr, n, d, s));
+    if (w.M16) HIPCHK(snn::launch_round_up_bf16(w.M, w.M16, (int64_t)nn, s));
   }
   if (timed) HIPCHK(hipEventRecord(e1, s));
 
@@ -599,6 +604,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     const bool pruned_step = pruned && (step > 0 || !fwd);
     if (pruned && !fwd && step > 0 && d - step <= m_refresh) {
       HIPCHK(snn::launch_max_square(w.Xc, st, w.M, n, pb.cpad, s));
+      if (w.M16) HIPCHK(snn::launch_round_up_bf16(w.M, w.M16, (int64_t)nn, s));
       m_refresh = (d - step) / 2;
     }
     // seeds: the previous step's update wrote them where it was fused
@@ -637,7 +643,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     if (step + 1 < pb.max_steps) {
       if (fused_seeds)
         HIPCHK(snn::launch_update_seeds(D, Dlo, w.XT, n, st, fwd ? 1 : -1, w.seeds, pb.npf,
-                                        sl.rows, w.tauw, w.frange, d, w.cand, w.iperm, w.M,
+                                        sl.rows, w.tauw, w.frange, d, w.cand, w.iperm, w.M16,
                                         s));
       else
         HIPCHK(snn::launch_update(D, Dlo, w.XT, n, st, -1, fwd ? 1 : -1, s));
@@ -800,7 +806,7 @@ void snn_destroy(snn_ctx* c) {
                     &c->kvals, &c->losses, &c->state, &c->scratch, &c->seeds,
                     &c->visited, &c->perm, &c->key, &c->M, &c->med, &c->skey,
                     &c->sidx, &c->sort_tmp, &c->rank, &c->tauw, &c->iperm, &c->frange,
-                    &c->lidx, &c->lval, &c->llb, &c->lcount};
+                    &c->lidx, &c->lval, &c->llb, &c->lcount, &c->M16};
   for (DevBuf* b : bufs) b->release();
   destroy_events(c);
   c->batch.drop_graphs();

@@ -18,12 +18,15 @@ hipError_t launch_max_square(const float* Xc, const SelState* st, float* M, int6
 // of the next step (seeds as launch_seed_scan writes them); n % 4 == 0.
 // rl.idx set also writes the next search's row lists (RowList) from the warm
 // bounds tauw (forward: and the chosen column's range frange; backward: the
-// lower bounds fl(D - M) from M).
+// lower bounds fl(D - M16) from M16 = M rounded up to bfloat16).
 hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
                                const SelState* st, int mult, int* seeds, int64_t npf,
                                const RowList& rl, const float* tauw, const float* frange, int d,
-                               const int* cand, const int* iperm, const float* M,
+                               const int* cand, const int* iperm, const unsigned short* M16,
                                hipStream_t s);
+// M16 = M rounded up to bfloat16, count values.
+hipError_t launch_round_up_bf16(const float* M, unsigned short* M16, int64_t count,
+                                hipStream_t s);
 hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const SelState* st,
                          int f, int mult, hipStream_t s);
 hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* Xc, int64_t n,

@@ -154,6 +154,31 @@ __global__ __launch_bounds__(256) void k_gram_exact(const float* __restrict__ XT
   }
 }
 
+// M16[i] = M[i] rounded up to bfloat16 (M >= 0; the carry of the rounding
+// moves into the exponent), the bound the distance update reads to list a
+// backward row's points: M16 >= M, so fl(D - M16) <= fl(D - M) is still a
+// lower bound of every remaining candidate's distance, at half the bytes.
+__global__ __launch_bounds__(256) void k_round_up_bf16(const float* __restrict__ M,
+                                                       unsigned short* __restrict__ M16,
+                                                       int64_t count) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x * 4;
+  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; i < count; i += stride) {
+    unsigned short h[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      uint32_t b = i + e < count ? __float_as_uint(M[i + e]) : 0u;
+      if (b & 0xFFFFu) b = (b + 0x10000u) & 0xFFFF0000u;
+      h[e] = (unsigned short)(b >> 16);
+    }
+    if (i + 3 < count) {
+      *reinterpret_cast<uint2*>(M16 + i) =
+          make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+    } else {
+      for (int e = 0; e < 4 && i + e < count; ++e) M16[i + e] = h[e];
+    }
+  }
+}
+
 // M refreshed over the remaining candidates of a backward selection:
 // M[i][j] = max over c < ncand of fl((Xc[i][c] - Xc[j][c])^2), the same
 // squares the search subtracts, so fl(D - M) stays a lower bound of every
@@ -297,6 +322,14 @@ __global__ __launch_bounds__(256) void k_update(float* __restrict__ D, float* __
   }
 }
 
+// Four bfloat16 values (8-byte aligned) widened to float, streamed once.
+__device__ __forceinline__ v4f bf16x4(const unsigned short* p) {
+  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+  const u2 w = __builtin_nontemporal_load(reinterpret_cast<const u2*>(p));
+  return v4f{__uint_as_float(w.x << 16), __uint_as_float(w.x & 0xFFFF0000u),
+             __uint_as_float(w.y << 16), __uint_as_float(w.y & 0xFFFF0000u)};
+}
+
 // The update fused with the next step's seed scan (n % 4 == 0, feature from
 // the device state): one wave per row i updates the row exactly as k_update
 // does and keeps, per lane, the two smallest new D_ij outside i's fold (lane
@@ -309,9 +342,9 @@ __global__ __launch_bounds__(256) void k_update(float* __restrict__ D, float* __
 // (cand[c < ncand], k_select has listed them): forward fl(max tau + Q)
 // widened as in the search's warm bound, backward max tau itself; no list
 // where some candidate's tau is unknown (kMaxF) or X is not finite.  The
-// backward lower bound fl(D - M) uses the M rows as they stand; a later
-// refresh only raises it, so the list still holds every point the search
-// can visit.
+// backward lower bound fl(D - M16) uses M rounded up to bfloat16 (M16 >= M),
+// from the rows as they stand; a later refresh only raises the bound, so the
+// list still holds every point the search can visit.
 template <bool COMP>
 __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
                                                       float* __restrict__ Dlo,
@@ -322,7 +355,7 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
                                                       const float* __restrict__ frange, int d,
                                                       const int* __restrict__ cand,
                                                       const int* __restrict__ iperm,
-                                                      const float* __restrict__ M) {
+                                                      const unsigned short* __restrict__ M16) {
   if (!st->pending) return;
   const int f = st->fstar;
   const int lane = threadIdx.x & 63;
@@ -364,7 +397,7 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
     int* __restrict__ lidx = rl.idx ? rl.idx + i * rl.cap : nullptr;
     float* __restrict__ lval = rl.idx ? rl.val + i * rl.cap : nullptr;
     float* __restrict__ llb = rl.lb ? rl.lb + i * rl.cap : nullptr;
-    const float* __restrict__ mrow = rl.lb ? M + i * n : nullptr;  // backward lists
+    const unsigned short* __restrict__ mrow = rl.lb ? M16 + i * n : nullptr;  // backward lists
     // in: the lane holds four points of the row (all lanes call this together);
     // mv: their M values (backward lists)
     v4f lbv = {0.f, 0.f, 0.f, 0.f};
@@ -427,8 +460,7 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
         xj[u] = *reinterpret_cast<const v4f*>(xf + j + u * 256);
         dv[u] = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j + u * 256));
         if (COMP) lv[u] = *reinterpret_cast<v4f*>(rlo + j + u * 256);
-        mv[u] = mrow ? __builtin_nontemporal_load(reinterpret_cast<const v4f*>(mrow + j + u * 256))
-                     : v4f{0.f, 0.f, 0.f, 0.f};
+        mv[u] = mrow ? bf16x4(mrow + j + u * 256) : v4f{0.f, 0.f, 0.f, 0.f};
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -446,7 +478,7 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
         xj = *reinterpret_cast<const v4f*>(xf + j);
         dv = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j));
         if (COMP) lv = *reinterpret_cast<v4f*>(rlo + j);
-        if (mrow) mv = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(mrow + j));
+        if (mrow) mv = bf16x4(mrow + j);
       }
       upd(dv, lv, xj, mv, j, in);
       if (in) {
@@ -1191,6 +1223,14 @@ hipError_t launch_gram_exact(const float* XT, float* D, float* Dlo, float* M, in
   return hipGetLastError();
 }
 
+hipError_t launch_round_up_bf16(const float* M, unsigned short* M16, int64_t count,
+                                hipStream_t s) {
+  const int64_t g = (count / 4 + 255) / 256;
+  hipLaunchKernelGGL(k_round_up_bf16, dim3((unsigned)(g < 16384 ? g : 16384)), dim3(256), 0, s,
+                     M, M16, count);
+  return hipGetLastError();
+}
+
 hipError_t launch_max_square(const float* Xc, const SelState* st, float* M, int64_t n, int cpad,
                              hipStream_t s) {
   dim3 g((unsigned)((n + 63) / 64), (unsigned)((n + 63) / 64));
@@ -1215,17 +1255,17 @@ hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const
 hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
                                const SelState* st, int mult, int* seeds, int64_t npf,
                                const RowList& rl, const float* tauw, const float* frange, int d,
-                               const int* cand, const int* iperm, const float* M,
+                               const int* cand, const int* iperm, const unsigned short* M16,
                                hipStream_t s) {
   if (n % 4) return hipErrorInvalidValue;
   const int64_t rows = (n + 3) / 4;
   const int g = (int)(rows < 8192 ? rows : 8192);
   if (Dlo)
     hipLaunchKernelGGL((k_update_seeds<true>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st, mult,
-                       seeds, npf, rl, tauw, frange, d, cand, iperm, M);
+                       seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
   else
     hipLaunchKernelGGL((k_update_seeds<false>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st, mult,
-                       seeds, npf, rl, tauw, frange, d, cand, iperm, M);
+                       seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
   return hipGetLastError();
 }
 
