@@ -193,6 +193,8 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     int rc = (q.X && q.y) ? snn::validate_problem(q.n, q.d, q.k_vals, q.num_k, q.loss_functions,
                                                   q.num_loss, &q.params, &pb)
                           : fail(SNN_EINVAL, "null input");
+    if (!rc && snn::search_list_size(pb.kmax) < 0)
+      rc = fail(SNN_ENOTSUP, "batched selections take k values up to 64 (snn_select takes more)");
     if (!rc && form >= 0 &&
         (q.params.form != form || (q.params.use_graph != 0) != (use_graph != 0)))
       rc = fail(SNN_EINVAL, "form and use_graph must agree across the batch");

@@ -76,7 +76,8 @@ int validate_problem(int64_t n, int d, const int* k_vals, int num_k, const int* 
     if (k_vals[m] < 1) return fail(SNN_EINVAL, "k values must be >= 1");
     kmax = k_vals[m] > kmax ? k_vals[m] : kmax;
   }
-  if (search_list_size(kmax) < 0) return fail(SNN_ENOTSUP, "k values above 64 are not supported");
+  if ((int64_t)kmax * 64 * 12 > ((int64_t)512 << 20))
+    return fail(SNN_ENOTSUP, "k values above 699050 are not supported");
   if (!loss_functions) return fail(SNN_EINVAL, "null loss function list");
   int rc = validate_losses(loss_functions, num_loss);
   if (rc) return rc;
@@ -335,6 +336,8 @@ struct Work {
   float *lval = nullptr, *llb = nullptr;
   unsigned short* M16 = nullptr;
   int *seeds = nullptr, *sidx = nullptr, *rank = nullptr;
+  float *long_d = nullptr, *long_lab = nullptr;
+  int* long_i = nullptr;
   unsigned long long* visited = nullptr;
   void* sort_tmp = nullptr;
 };
@@ -432,8 +435,12 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   // precision form: exact pruned search over D in the reference's bits; the
   // compensation term Dlo of D is kept only on request (nothing in the
   // selection reads it)
-  const bool pruned = prm->form == SNN_FORM_PRECISION;
-  const bool comp = pruned && prm->compensated;
+  // k values above 64: one search kernel with its lists in global memory
+  // serves every step of either form (the pruned kernels keep their lists in
+  // registers)
+  const bool long_list = snn::search_list_size(kmax) < 0;
+  const bool pruned = prm->form == SNN_FORM_PRECISION && !long_list;
+  const bool comp = prm->form == SNN_FORM_PRECISION && prm->compensated;
   const bool timed = arena == nullptr;
 
   HIPCHK(hipSetDevice(ctx->device));
@@ -499,6 +506,13 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   // median losses: sort space for NaN-holding error vectors
   if (median) bd.add(ctx->med, sizeof(float) * 2 * (size_t)n * pb.cpad * num_k, &w.med);
   if (candidate_errors) bd.add(ctx->cand_err, sizeof(float) * (size_t)d * d, &w.cand_err);
+  const int64_t long_waves = long_list ? snn::long_search_waves(n, pb.cpad, kmax) : 0;
+  if (long_list) {
+    const size_t slots = (size_t)long_waves * (size_t)kmax * 64;
+    bd.add(ctx->long_d, sizeof(float) * slots, &w.long_d);
+    bd.add(ctx->long_i, sizeof(int) * slots, &w.long_i);
+    bd.add(ctx->long_lab, sizeof(float) * slots, &w.long_lab);
+  }
   rc = bd.bind();
   if (rc) return rc;
   if (timed) {
@@ -621,6 +635,8 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
       int rc2 = search_pruned_tuned(sl, d, step, rank, s, &used_variant,
                                     timed ? ep.tune : nullptr);
       if (rc2) return rc2;
+    } else if (long_list) {
+      HIPCHK(snn::launch_search_long(sl, w.long_d, w.long_i, w.long_lab, long_waves, s));
     } else {
       // first forward step (D = 0): sorted columns; the dense search runs
       // instead where X is not finite (each kernel checks the state's flag)

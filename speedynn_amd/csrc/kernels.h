@@ -32,6 +32,12 @@ hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const
 hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* Xc, int64_t n,
                           int d, int cpad, hipStream_t s);
 hipError_t launch_search_dense(const SearchProblem& a, hipStream_t s);
+// Lists longer than 64 (a.kmax > 64): global-memory lists of a.kmax slots for
+// `waves` persistent waves (ld, li, lab: waves * a.kmax * 64 values each;
+// long_search_waves gives the count for a shape).
+int64_t long_search_waves(int64_t n, int cpad, int K);
+hipError_t launch_search_long(const SearchProblem& a, float* ld, int* li, float* lab,
+                              int64_t waves, hipStream_t s);
 // variant: 0 = by direction (forward: a wave per (query, candidate block);
 // backward: a workgroup per query sharing the row scan), 1 = per wave,
 // 2 = shared scan (where pruned_shared_applies), 3 = shared scan split over

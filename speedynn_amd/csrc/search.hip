@@ -1269,7 +1269,92 @@ __global__ __launch_bounds__(256) void k_seed_scan(const float* __restrict__ D,
   sp.write_ranked(seeds + q * 128);
 }
 
+// Search with lists longer than the register kernels compile (k values above
+// 64; the reference takes any k, speedynn/lib/cpu.c:50-51).  One lane per
+// (query, candidate) as in dense_block, but the list of K (distance, index)
+// pairs lives in global memory, slot m of a lane at [wave][m][lane], and a
+// point enters by the reference's own steps (cpu.c:149-165): strict '<'
+// against the last slot, then it moves toward the front while strictly below
+// its neighbour, so equal distances keep the lower index.  The labels of the
+// final list go beside it for predict().  Persistent: wave w of W serves the
+// items (query, candidate block) w, w + W, ...
+__global__ __launch_bounds__(256) void k_search_long(SearchProblem a, int K, float* __restrict__ ld,
+                                                     int* __restrict__ li, float* __restrict__ lab_g,
+                                                     int64_t waves_total) {
+  if (a.st->stopped) return;
+  const int ncand = a.st->ncand;
+  const int nblk = (ncand + 63) / 64;
+  const int lane = threadIdx.x & 63;
+  const int64_t wid = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wid >= waves_total) return;
+  const int64_t base = wid * (int64_t)K * 64 + lane;
+  float* Ld = ld + base;  // slot m at Ld[m * 64]
+  int* Li = li + base;
+  float* lab = lab_g + base;
+  const int64_t n = a.n, npf = a.npf;
+  const int cpad = a.cpad;
+  const int64_t items = n * nblk;
+  for (int64_t it = wid; it < items; it += waves_total) {
+    const int64_t q = it / nblk;
+    const int c = (int)(it - q * nblk) * 64 + lane;
+    const bool live = c < ncand;
+    const float xq = a.Xc[q * cpad + c];
+    for (int m = 0; m < K; ++m) {
+      Ld[m * 64] = kMaxF;
+      Li[m * 64] = 0;
+    }
+    float last = kMaxF;
+    const int64_t flo = (q / npf) * npf, fhi = flo + npf;
+    const float* __restrict__ Drow = a.D + q * n;
+    const float* __restrict__ xc = a.Xc + c;
+    for (int64_t j = 0; j < n; ++j) {
+      if (j == flo) {  // the query's fold: cpu.c:139
+        j = fhi - 1;
+        continue;
+      }
+      const float t = xq - xc[j * cpad];
+      const float p = t * t;
+      const float dd = a.mult > 0 ? Drow[j] + p : Drow[j] - p;
+      if (dd < last) {
+        int m = K - 1;
+        for (; m > 0; --m) {
+          const float prev = Ld[(m - 1) * 64];
+          if (!(dd < prev)) break;
+          Ld[m * 64] = prev;
+          Li[m * 64] = Li[(m - 1) * 64];
+        }
+        Ld[m * 64] = dd;
+        Li[m * 64] = (int)j;
+        last = Ld[(K - 1) * 64];
+      }
+    }
+    for (int m = 0; m < K; ++m) lab[m * 64] = a.y[Li[m * 64]];
+    for (int m = 0; m < a.nk; ++m) {
+      const float pred = predict(lab, a.kvals[m], a.model_type);
+      if (live) a.P[((int64_t)m * cpad + c) * n + q] = pred;
+    }
+  }
+}
+
 // ============================================================ launchers
+
+int64_t long_search_waves(int64_t n, int cpad, int K) {
+  // enough waves to fill the device several times over, with the lists
+  // (12 bytes per slot and lane) within 512 MiB
+  int64_t w = n * (cpad / 64);
+  const int64_t cap = ((int64_t)512 << 20) / ((int64_t)K * 64 * 12);
+  if (w > 8192) w = 8192;
+  if (w > cap) w = cap;
+  return w < 1 ? 1 : w;
+}
+
+hipError_t launch_search_long(const SearchProblem& a, float* ld, int* li, float* lab,
+                              int64_t waves, hipStream_t s) {
+  const int K = a.kmax;
+  const dim3 g((unsigned)((waves + 3) / 4)), b(256);
+  hipLaunchKernelGGL(k_search_long, g, b, 0, s, a, K, ld, li, lab, waves);
+  return hipGetLastError();
+}
 
 template <int K, int QW, int WAVES>
 static hipError_t launch_dense_k(const SearchProblem& a, hipStream_t s) {

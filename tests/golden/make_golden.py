@@ -90,6 +90,20 @@ def cases():
     raw("cls_n777_fwd_auto", X5, y5, k_vals=[5, 10, 20], num_features=3, k_fold=777,
         loss_functions=["zero_one"], auto_feature=True, model_type="classification",
         method="forward")
+    # lists longer than the 64 slots of the register kernels, one of them longer than
+    # the eligible points (the unfilled slots keep (MAX_FLOAT, index 0), so label 0 votes)
+    X7, y7, _ = ds.classification_argmax(13, 300, 20, 6, 3, 0.05)
+    raw("cls3_fwd_long_lists", X7, y7, k_vals=[70, 100, 3], num_features=3, k_fold=300,
+        loss_functions=["zero_one", "rmse"], auto_feature=False, model_type="classification",
+        method="forward")
+    X8, y8, _ = ds.regression_sum(14, 200, 12, 3, 0.5)
+    raw("reg_bwd_long_lists_kfold", X8, y8, k_vals=[65, 150], num_features=9, k_fold=7,
+        loss_functions=["rmse", "mad"], auto_feature=False, model_type="regression",
+        method="backward")
+    X9, y9, _ = ds.classification_sign(15, 90, 6, 2, 0.1)
+    raw("cls_k_above_eligible", X9, y9, k_vals=[100, 89], num_features=2, k_fold=90,
+        loss_functions=["zero_one"], auto_feature=False, model_type="classification",
+        method="forward")
     X6, y6, _ = ds.regression_sum(12, 500, 24, 4, 0.5)
     raw("reg_n500_bwd_k15", X6, y6, k_vals=[15], num_features=4, k_fold=500,
         loss_functions=["rmse"], auto_feature=False, model_type="regression", method="backward")
@@ -118,7 +132,10 @@ def main():
     O.build(with_ref=True)
     if not O.ref_available():
         raise SystemExit("the reference module could not be built")
+    only = set(sys.argv[1:])  # case names to write (default: every case)
     for name, inp, X, y in cases():
+        if only and name not in only:
+            continue
         args = (X, y, inp["k_vals"], inp["num_features"], inp["k_fold"], inp["loss_functions"],
                 inp["auto_feature"], inp["model_type"], inp["method"])
         r = O.ref_select(*args)

@@ -89,6 +89,34 @@ def test_every_candidate_error_matches_oracle(engine, case, form):
                                   o["candidate_errors"][:steps].view(np.uint32))
 
 
+LONG = [
+    # lists longer than 64 (global-memory lists): forward LOO, backward over uneven folds,
+    # and k above the eligible points (unfilled slots keep index 0); two candidate blocks
+    (71, 300, 10, [70, 100, 3], 300, "classification", "forward", 4),
+    (72, 257, 9, [65, 3], 13, "regression", "backward", 4),
+    (73, 90, 70, [100, 5], 90, "classification", "forward", 2),
+]
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["plain", "precision"])
+@pytest.mark.parametrize("case", LONG, ids=lambda c: f"seed{c[0]}")
+def test_lists_longer_than_64_match_oracle(engine, case, form):
+    """k values above 64 (the reference takes any k, cpu.c:50-51): every
+    candidate's error at every step against the oracle, bit for bit."""
+    seed, n, d, k_vals, k_fold, model, method, nf = case
+    X, y = _random_case(seed, n, d, model)
+    loss = "zero_one" if model == "classification" else "rmse"
+    print(f"long-list case seed={seed} n={n} d={d} k={k_vals} form={form}", flush=True)
+    o = O.select(X, y, k_vals, nf, k_fold, [loss], False, model, method, record_candidates=True)
+    inp = dict(k_vals=k_vals, loss_functions=[loss], method=method, model_type=model,
+               k_fold=k_fold, num_features=nf, auto_feature=False)
+    g = run(engine, X, y, inp, form, candidate_errors=True)
+    assert_same_result(g, o)
+    steps = g["steps"]
+    np.testing.assert_array_equal(g["candidate_errors"][:steps].view(np.uint32),
+                                  o["candidate_errors"][:steps].view(np.uint32))
+
+
 NONFINITE = [
     # seed, n, d, k_vals, k_fold, model, method, num_features
     (63, 400, 10, [5, 10, 20], 400, "regression", "forward", 4),
