@@ -89,18 +89,41 @@ __device__ __forceinline__ void finish_query(const SearchProblem& a, const int (
 
 // Dense search of one item: queries [qb*WAVES*QW, +WAVES*QW) x candidate
 // block cb, one lane per (query, candidate): a wave owns QW queries and the 64
-// candidates.  Per step of 64 points the block stages the candidates' columns
-// Xc[j0:j0+64][cb*64:+64] in LDS (double-buffered), so each staged value
-// serves WAVES*QW queries; the query's D row is wave-uniform and read with
-// scalar loads.  Per point: d = fl(D_qj + mult*fl((x_qf - x_jf)^2))
-// (speedynn/lib/cpu.c:142-144), points in the query's fold are skipped
-// (cpu.c:139: LOO when k_fold == n).  ZERO: D is known to be 0 (first forward
-// step), so d = fl((x_qf - x_jf)^2) exactly and D is not read.  Called by the
-// whole block; returns early only block-uniformly.
+// candidates.  Per chunk of 64 points the block stages the candidates' values
+// Xc[j0:j0+64][cb*64:+64] and its queries' D values in LDS, double-buffered:
+// the next chunk's loads are in flight while this one is scanned, and each
+// staged value serves WAVES*QW queries.  Per point:
+// d = fl(D_qj + mult*fl((x_qf - x_jf)^2)) (speedynn/lib/cpu.c:142-144); points
+// in the query's fold are skipped (cpu.c:139: LOO when k_fold == n).  ZERO: D
+// is known to be 0 (first forward step), so d = fl((x_qf - x_jf)^2) exactly
+// and D is not read.
+// A point whose d is below the lane's threshold (its list's K-th distance at
+// the last flush) goes to the lane's pending buffer in LDS.  When some lane's
+// buffer may overflow, the wave inserts every lane's entries in arrival
+// order, i.e. ascending index, each tested strictly against the current list,
+// so the lists equal those of offering every point in index order, and the
+// K-slot insertion network runs once per entry of the busiest lane instead of
+// once per point on which any of the 64 lanes inserts.
+// Called by the whole block; returns early only block-uniformly.
+constexpr int kDensePend = 12;  // pending (distance, index) slots per lane and query
+constexpr int kDenseBatch = 4;  // points per scan batch (one 16-byte D read)
+
+template <int K, int QW, int WAVES, bool ZERO>
+constexpr int dense_lds_words() {
+  constexpr int scan = 2 * 64 * 64 + (ZERO ? 0 : 2 * WAVES * QW * 64) +
+                       WAVES * QW * 2 * kDensePend * 64;
+  constexpr int epi = WAVES * K * 64;
+  return scan > epi ? scan : epi;
+}
+
 template <int K, int QW, int WAVES, int MULT, bool ZERO>
 __device__ __forceinline__ void dense_block(const SearchProblem& a, float* lds, int64_t qb,
                                             int cb) {
-  constexpr int J = 64;
+  constexpr int J = 64, S = kDensePend, B = kDenseBatch;
+  constexpr int NQ = WAVES * QW;               // queries of the block
+  constexpr int XV = J * 16 / (64 * WAVES);    // float4 of the X tile per thread
+  static_assert(XV * 64 * WAVES == J * 16, "X tile split");
+  static_assert(B == 4, "one float4 of D per batch");
   if (a.st->stopped) return;
   if (ZERO && a.sorted && !a.st->nonfinite) return;  // k_search_sorted serves this step
   const int ncand = a.st->ncand;
@@ -110,22 +133,26 @@ __device__ __forceinline__ void dense_block(const SearchProblem& a, float* lds, 
   const int lane = threadIdx.x & 63;
   const int wave = uniform(threadIdx.x >> 6);
   const int c = cb * 64 + lane;
-  const int64_t qbase = (qb * WAVES + wave) * QW;
+  const int64_t qbase = qb * NQ + wave * QW;  // this wave's first query
 
-  float xq[QW];
+  float* xt = lds;                  // [2][J][64]: X tiles
+  float* dt = lds + 2 * J * 64;     // [2][NQ][J]: D tiles
+  float* pend = dt + (ZERO ? 0 : 2 * NQ * J) + wave * QW * (2 * S * 64) + lane;
+
+  float xq[QW], thr[QW];
   float Ld[QW][K];
   int Li[QW][K];
+  int pc[QW];
   int64_t flo[QW], fhi[QW];
-  const float* __restrict__ Drow[QW];
 #pragma unroll
   for (int qi = 0; qi < QW; ++qi) {
     const int64_t q = qbase + qi;
-    // queries past n (last item) scan row n-1 and never store a result
-    Drow[qi] = a.D + (q < n ? q : n - 1) * n;
     xq[qi] = q < n ? a.Xc[q * cpad + c] : 0.0f;
     const int64_t fold = q / npf;
     flo[qi] = q < n ? fold * npf : n;
     fhi[qi] = q < n ? (fold + 1) * npf : n;
+    thr[qi] = kMaxF;
+    pc[qi] = 0;
 #pragma unroll
     for (int m = 0; m < K; ++m) {
       Ld[qi][m] = kMaxF;
@@ -133,69 +160,126 @@ __device__ __forceinline__ void dense_block(const SearchProblem& a, float* lds, 
     }
   }
 
+  // the next chunk's values, loaded at the top of an iteration and written to
+  // LDS at its end
+  float4 xr[XV];
+  float dr[QW];
   const float* __restrict__ Xblk = a.Xc + cb * 64;
-  auto stage = [&](int buf, int64_t j0) {
-    float* dst = lds + buf * (J * 64);
-    for (int t = threadIdx.x; t < J * 16; t += 64 * WAVES) {
-      const int r = t >> 4, c4 = (t & 15) * 4;
-      const int64_t j = j0 + r;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (j < n) v = *reinterpret_cast<const float4*>(Xblk + j * cpad + c4);
-      *reinterpret_cast<float4*>(dst + r * 64 + c4) = v;
+  auto fetch = [&](int64_t j0) {
+#pragma unroll
+    for (int v = 0; v < XV; ++v) {
+      const int t = threadIdx.x + v * 64 * WAVES;
+      const int64_t j = j0 + (t >> 4);
+      xr[v] = j < n ? *reinterpret_cast<const float4*>(Xblk + j * cpad + (t & 15) * 4)
+                    : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (!ZERO) {
+      const int64_t j = j0 + lane;
+#pragma unroll
+      for (int qi = 0; qi < QW; ++qi) {
+        const int64_t q = qbase + qi;
+        dr[qi] = j < n ? a.D[(q < n ? q : n - 1) * n + j] : 0.0f;
+      }
+    }
+  };
+  auto commit = [&](int b) {
+    float* dst = xt + b * (J * 64);
+#pragma unroll
+    for (int v = 0; v < XV; ++v) {
+      const int t = threadIdx.x + v * 64 * WAVES;
+      *reinterpret_cast<float4*>(dst + (t >> 4) * 64 + (t & 15) * 4) = xr[v];
+    }
+    if (!ZERO) {
+#pragma unroll
+      for (int qi = 0; qi < QW; ++qi) dt[(b * NQ + wave * QW + qi) * J + lane] = dr[qi];
+    }
+  };
+  auto flush = [&](int qi) {
+    int top = pc[qi];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) top = max(top, __shfl_xor(top, o));
+    const float* pb = pend + qi * (2 * S * 64);
+    for (int sl = 0; sl < top; ++sl) {
+      if (sl < pc[qi]) {
+        const float dd = pb[sl * 64];
+        const int jj = __float_as_int(pb[(S + sl) * 64]);
+        if (dd < Ld[qi][K - 1]) topk_insert<K>(Ld[qi], Li[qi], dd, jj);
+      }
+    }
+    pc[qi] = 0;
+    thr[qi] = Ld[qi][K - 1];
+  };
+  // one batch of B points from jj of the chunk at j0; CHECK: mask points
+  // past n and in a query's fold
+  auto batch = [&](const float* xs, const float* dsq, int64_t j0, int jj, bool check) {
+    float xj[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) xj[u] = xs[(jj + u) * 64];
+#pragma unroll
+    for (int qi = 0; qi < QW; ++qi) {
+      float dv[B] = {0.f, 0.f, 0.f, 0.f};
+      if (!ZERO) {
+        const float4 d4 = *reinterpret_cast<const float4*>(dsq + qi * J + jj);
+        dv[0] = d4.x;
+        dv[1] = d4.y;
+        dv[2] = d4.z;
+        dv[3] = d4.w;
+      }
+      float dd[B];
+      bool p[B];
+      bool any = false;
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        const float t = xq[qi] - xj[u];
+        const float pp = t * t;
+        dd[u] = ZERO ? pp : (MULT > 0 ? dv[u] + pp : dv[u] - pp);
+        p[u] = dd[u] < thr[qi];
+        if (check) {
+          const int64_t j = j0 + jj + u;
+          p[u] = p[u] && j < n && (j < flo[qi] || j >= fhi[qi]);
+        }
+        any |= p[u];
+      }
+      if (__ballot(any)) {
+        float* pb = pend + qi * (2 * S * 64);
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+          if (p[u]) {
+            pb[pc[qi] * 64] = dd[u];
+            pb[(S + pc[qi]) * 64] = __int_as_float((int)(j0 + jj + u));
+            ++pc[qi];
+          }
+        }
+        if (__ballot(pc[qi] > S - B)) flush(qi);
+      }
     }
   };
 
   __syncthreads();  // the previous item of a persistent block may still read lds
-  stage(0, 0);
+  fetch(0);
+  commit(0);
   __syncthreads();
   int buf = 0;
   for (int64_t j0 = 0; j0 < n; j0 += J) {
-    if (j0 + J < n) stage(buf ^ 1, j0 + J);
-    const float* xs = lds + buf * (J * 64);
-    const int jn = (int)((n - j0) < J ? (n - j0) : J);
-    bool skip_any = false;
+    const bool more = j0 + J < n;
+    if (more) fetch(j0 + J);
+    const float* xs = xt + buf * (J * 64) + lane;
+    const float* dsq = dt + (buf * NQ + wave * QW) * J;
+    bool check = n - j0 < J;
 #pragma unroll
-    for (int qi = 0; qi < QW; ++qi)
-      skip_any |= (flo[qi] < j0 + jn) && (fhi[qi] > j0);
-    if (!skip_any && jn == J) {
-      // no fold overlap in this chunk: unchecked inner loop
-      for (int jj = 0; jj < J; jj += 4) {
-        float dq[QW][4];
-#pragma unroll
-        for (int qi = 0; qi < QW; ++qi)
-#pragma unroll
-          for (int u = 0; u < 4; ++u) dq[qi][u] = ZERO ? 0.0f : Drow[qi][j0 + jj + u];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const float xj = xs[(jj + u) * 64 + lane];
-          const int j = (int)(j0 + jj + u);
-#pragma unroll
-          for (int qi = 0; qi < QW; ++qi) {
-            const float t = xq[qi] - xj;
-            const float p = t * t;
-            const float dd = ZERO ? p : (MULT > 0 ? dq[qi][u] + p : dq[qi][u] - p);
-            if (dd < Ld[qi][K - 1]) topk_insert<K>(Ld[qi], Li[qi], dd, j);
-          }
-        }
-      }
+    for (int qi = 0; qi < QW; ++qi) check |= (flo[qi] < j0 + J) && (fhi[qi] > j0);
+    if (check) {
+      for (int jj = 0; jj < J; jj += B) batch(xs, dsq, j0, jj, true);
     } else {
-      for (int jj = 0; jj < jn; ++jj) {
-        const float xj = xs[jj * 64 + lane];
-        const int64_t j = j0 + jj;
-#pragma unroll
-        for (int qi = 0; qi < QW; ++qi) {
-          if (j >= flo[qi] && j < fhi[qi]) continue;
-          const float dqv = ZERO ? 0.0f : Drow[qi][j];
-          const float t = xq[qi] - xj;
-          const float p = t * t;
-          const float dd = ZERO ? p : (MULT > 0 ? dqv + p : dqv - p);
-          if (dd < Ld[qi][K - 1]) topk_insert<K>(Ld[qi], Li[qi], dd, (int)j);
-        }
-      }
+      for (int jj = 0; jj < J; jj += B) batch(xs, dsq, j0, jj, false);
     }
+    if (more) commit(buf ^ 1);
     __syncthreads();
     buf ^= 1;
   }
+#pragma unroll
+  for (int qi = 0; qi < QW; ++qi) flush(qi);
+  __syncthreads();  // tiles and pending areas are read; the labels reuse them
 
   float* lab = lds + wave * (K * 64) + lane;
   const bool live = c < ncand;
@@ -766,9 +850,7 @@ __global__ __launch_bounds__(256) void k_search_sorted(const SearchProblem a) {
 
 template <int K, int QW, int WAVES, int MULT, bool ZERO>
 __global__ __launch_bounds__(64 * WAVES) void k_search_dense(const SearchProblem a) {
-  constexpr int LDS_SCAN = 2 * 64 * 64;
-  constexpr int LDS_EPI = WAVES * K * 64;
-  __shared__ float lds[LDS_SCAN > LDS_EPI ? LDS_SCAN : LDS_EPI];
+  __shared__ __align__(16) float lds[dense_lds_words<K, QW, WAVES, ZERO>()];
   dense_block<K, QW, WAVES, MULT, ZERO>(a, lds, blockIdx.x, blockIdx.y);
 }
 
@@ -1197,9 +1279,7 @@ template <int K, int QW, int WAVES, int MULT, bool ZERO>
 __global__ __launch_bounds__(64 * WAVES) void k_batch_dense(
     const SearchProblem* __restrict__ probs, const int64_t* __restrict__ start, int P,
     unsigned long long* __restrict__ counter) {
-  constexpr int LDS_SCAN = 2 * 64 * 64;
-  constexpr int LDS_EPI = WAVES * K * 64;
-  __shared__ float lds[LDS_SCAN > LDS_EPI ? LDS_SCAN : LDS_EPI];
+  __shared__ __align__(16) float lds[dense_lds_words<K, QW, WAVES, ZERO>()];
   __shared__ long long s_item;
   const int64_t total = start[P];
   for (;;) {
@@ -1433,8 +1513,8 @@ static hipError_t launch_pruned_k(const SearchProblem& a, int variant, hipStream
 }
 
 #define SNN_FOR_EACH_LIST_SIZE(X) \
-  X(2, 2, 8) X(4, 2, 8) X(5, 2, 8) X(8, 2, 8) X(10, 2, 8) X(15, 2, 8) X(16, 2, 8) X(20, 2, 8) \
-  X(24, 2, 8) X(32, 1, 8) X(50, 1, 4) X(64, 1, 4)
+  X(2, 1, 16) X(4, 1, 16) X(5, 1, 16) X(8, 1, 16) X(10, 1, 16) X(15, 1, 16) X(16, 1, 16) \
+  X(20, 1, 16) X(24, 1, 8) X(32, 1, 8) X(50, 1, 4) X(64, 1, 4)
 
 hipError_t launch_search_dense(const SearchProblem& a, hipStream_t s) {
   switch (search_list_size(a.kmax)) {
