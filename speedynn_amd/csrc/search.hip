@@ -494,8 +494,8 @@ constexpr int pend_slots() {
 
 // Words of LDS one wave of the per-wave pruned kernels uses: the pending
 // buffer during the scan, the epilogue labels (K slots x 64 lanes) after it,
-// and then the ring of visited points (kRing (index, D) pairs, from word
-// ring_offset<K>()).
+// and then the ring of visited points (kRing (offset, D) pairs from word
+// ring_offset<K>(), then their kRing point indices).
 constexpr int kRing = 128;
 template <int K>
 constexpr int ring_offset() {
@@ -503,7 +503,7 @@ constexpr int ring_offset() {
 }
 template <int K>
 constexpr int wave_lds_words() {
-  return ring_offset<K>() + 2 * kRing;
+  return ring_offset<K>() + 3 * kRing;
 }
 
 // Flush of the pending buffer `buf` (the lane's slot 0; `pc` entries): insert
@@ -624,22 +624,24 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
   };
   const uint32_t ucpad = (uint32_t)cpad;  // n * cpad < 2^31 (checked on the host)
   // Visits.  Each chunk's points to visit (the set bits of its ballot) are
-  // appended, in ascending point order, to a ring of (index, D) pairs in the
-  // wave's LDS area by one lane-parallel store, and evaluated eight at a time
-  // from the ring's head: four broadcast 16-byte LDS reads, the gathers of the
-  // 64 candidates' values (256 B each, coalesced) and the screen, with the
-  // batch's bookkeeping in a few scalar instructions, and batches that stay
-  // full across chunk boundaries.  At most 7 + 64 entries are pending, so the
-  // ring (128) never overflows and full batches start at multiples of 8.  The
-  // order of evaluation is the order of the points, so the pending buffer
-  // still receives ascending indices.
+  // appended, in ascending point order, to a ring of (offset j*cpad, D) pairs
+  // in the wave's LDS area (and their indices j beside it) by one
+  // lane-parallel store, and evaluated eight at a time from the ring's head:
+  // four broadcast 16-byte LDS reads whose values the gathers and the screen
+  // use as they are (no move to scalar registers), the gathers of the 64
+  // candidates' values (256 B each, coalesced) and the screen; the index is
+  // read only for a point that passes.  Batches stay full across chunk
+  // boundaries.  At most 7 + 64 entries are pending, so the ring (128) never
+  // overflows and full batches start at multiples of 8.  The order of
+  // evaluation is the order of the points, so the pending buffer still
+  // receives ascending indices.
   int2* __restrict__ ring = reinterpret_cast<int2*>(buf - lane + ring_offset<K>());
+  int* __restrict__ ringj = reinterpret_cast<int*>(buf - lane + ring_offset<K>() + 2 * kRing);
   int rhead = 0, rtail = 0;  // wave-uniform: entries [rhead, rtail) are pending
   constexpr int BATCH = 8;
   // screen of a batch (cnt valid entries); the exact test runs only if some
   // lane may pass
-  auto screen = [&](const int (&js)[BATCH], const float (&dqs)[BATCH], float (&ts)[BATCH],
-                    int cnt) {
+  auto screen = [&](int base, const float (&dqs)[BATCH], float (&ts)[BATCH], int cnt) {
     bool pass[BATCH];
     bool any = false;
 #pragma unroll
@@ -656,7 +658,7 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
           const float dd = dist(dqs[u], ts[u] * ts[u]);
           if (dd < Ld[K - 1]) {
             buf[pc * 64] = dd;
-            buf[(pend_slots<K>() + pc) * 64] = __int_as_float(js[u]);
+            buf[(pend_slots<K>() + pc) * 64] = __int_as_float(ringj[(base + u) & (kRing - 1)]);
             ++pc;
           }
         }
@@ -669,33 +671,33 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
     int4 e[4];
 #pragma unroll
     for (int h = 0; h < 4; ++h) e[h] = r4[h];  // broadcast reads, all issued first
-    int js[BATCH];
+    uint32_t os[BATCH];
     float dqs[BATCH], ts[BATCH];
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
-      js[2 * h] = __builtin_amdgcn_readfirstlane(e[h].x);
-      dqs[2 * h] = __int_as_float(__builtin_amdgcn_readfirstlane(e[h].y));
-      js[2 * h + 1] = __builtin_amdgcn_readfirstlane(e[h].z);
-      dqs[2 * h + 1] = __int_as_float(__builtin_amdgcn_readfirstlane(e[h].w));
+      os[2 * h] = (uint32_t)e[h].x;
+      dqs[2 * h] = __int_as_float(e[h].y);
+      os[2 * h + 1] = (uint32_t)e[h].z;
+      dqs[2 * h + 1] = __int_as_float(e[h].w);
     }
 #pragma unroll
-    for (int u = 0; u < BATCH; ++u) ts[u] = Xl[(uint32_t)js[u] * ucpad];
+    for (int u = 0; u < BATCH; ++u) ts[u] = Xl[os[u]];
+    const int base = rhead;
     rhead += BATCH;
-    screen(js, dqs, ts, BATCH);
+    screen(base, dqs, ts, BATCH);
   };
   auto eval_tail = [&]() {  // the last (< 8) entries
     const int cnt = rtail - rhead;
-    int js[BATCH];
     float dqs[BATCH], ts[BATCH];
 #pragma unroll
     for (int u = 0; u < BATCH; ++u) {
       const int2 e = ring[(rhead + (u < cnt ? u : 0)) & (kRing - 1)];
-      js[u] = __builtin_amdgcn_readfirstlane(e.x);
-      dqs[u] = __int_as_float(__builtin_amdgcn_readfirstlane(e.y));
-      ts[u] = u < cnt ? Xl[(uint32_t)js[u] * ucpad] : 0.0f;
+      dqs[u] = __int_as_float(e.y);
+      ts[u] = u < cnt ? Xl[(uint32_t)e.x] : 0.0f;
     }
+    const int base = rhead;
     rhead = rtail;
-    screen(js, dqs, ts, cnt);
+    screen(base, dqs, ts, cnt);
   };
   // a chunk's points to visit: lane l holds point jv with D value dcur, and
   // bit l of mask says whether to visit it
@@ -703,7 +705,9 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
     if ((mask >> lane) & 1ull) {
       const int r = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-      ring[(rtail + r) & (kRing - 1)] = make_int2(jv, __float_as_int(dcur));
+      const int pos = (rtail + r) & (kRing - 1);
+      ring[pos] = make_int2((int)((uint32_t)jv * ucpad), __float_as_int(dcur));
+      ringj[pos] = jv;
     }
     __builtin_amdgcn_wave_barrier();  // the stores above precede the reads of other lanes
     rtail += (int)__popcll(mask);
