@@ -99,7 +99,7 @@ def cpu_baseline(X, y, kv, k_fold, model_type, method, target_s: float = 12.0) -
     step = "first forward step (D=0)" if mult > 0 else "first backward step (full D)"
     return {"value": nq * d / dt, "unit": "evals/s", "cores": threads, "kind": "port",
             "sample": f"CPU oracle, {step}: {nq} queries x {d} candidates x {n} points, "
-                      f"{dt:.1f} s on {threads} threads"}
+                      f"{dt:.2f} s on {threads} threads"}
 
 
 def roofline(alg_bytes: float, ms: float, launches: int, kernel: str, traffic, unit_note: str):
@@ -174,8 +174,8 @@ def bench_single(args, hip, eng, form: int) -> dict:
     kw = dict(method=method, model_type=mt, k_fold=w["k_fold"], num_features=w["num_features"],
               device_inputs=(xp, yp, n, d))
 
-    def once(f=form):
-        return eng.select(None, None, w["k_vals"], [loss], form=f, **kw)
+    def once(f=form, graph=False):
+        return eng.select(None, None, w["k_vals"], [loss], form=f, use_graph=graph, **kw)
 
     cold = None  # the process's first selection: the autotuner times its kernels here
     for i in range(args.warmup):
@@ -258,6 +258,21 @@ def bench_single(args, hip, eng, form: int) -> dict:
         "selected": (r["selected_ordering"][:steps_sel].tolist() if method == 1 else
                      r["selected_ordering"][::-1][:min(steps_sel, 32)].tolist()),
     }
+    if not args.no_graph:
+        # the same selection with its steps replayed from hipGraphs (one call
+        # captures them), outside the timed region; graph mode keeps no phase
+        # times, so the roofline above comes from the stream-issued steps
+        once(graph=True)
+        hip.sync()
+        ng = min(args.steps, 3)
+        t1 = time.perf_counter()
+        gr = [once(graph=True) for _ in range(ng)]
+        hip.sync()
+        line["graph_selection_wall_s"] = (time.perf_counter() - t1) / ng
+        line["graph_equals_stream"] = all(
+            np.array_equal(g["selected_ordering"], r["selected_ordering"]) and
+            np.array_equal(g["selected_errors"].view(np.uint32),
+                           r["selected_errors"].view(np.uint32)) for g in gr)
     if form == 1 and not args.no_plain:
         # the plain FP32 form (dense search every step) once, outside the
         # timed region: its wall time and whether the precision form's
@@ -423,6 +438,8 @@ def main() -> int:
     ap.add_argument("--no-sequential", action="store_true", help="c3: skip the sequential runs")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the one plain-form selection beside the precision form")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the selections replayed from step graphs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="run device-to-device copies of BYTES (counter calibration) and exit")
     ap.add_argument("--gram", choices=["c1", "c2", "c4"], default=None,

@@ -93,7 +93,11 @@ typedef struct {
     int num_features;    /* desired features (ignored when auto_feature) */
     int auto_feature;    /* stop at the first error increase */
     int form;            /* SNN_FORM_* */
-    int use_graph;       /* snn_select_batch: capture one selection step in a hipGraph */
+    int use_graph;       /* capture selection steps in hipGraphs and replay them:
+                            snn_select_batch one graph per step kind; snn_select and
+                            snn_select_device one graph per distinct launch sequence
+                            of a step, kept in the context for later calls of the same
+                            shape (no phase times in the summary then) */
     int compensated;     /* precision form: also keep D's compensation term (TwoSum /
                             TwoProduct error, one more n*n float array); the selection
                             does not depend on it */

@@ -149,10 +149,12 @@ class Engine:
     def select(self, X, y, k_vals, loss_codes, *, method: int, model_type: int, k_fold: int,
                num_features: int, auto_feature: bool = False, form: int = 1,
                candidate_errors: bool = False, device_inputs=None,
-               compensated: bool = False, noalloc: bool = False) -> dict:
+               compensated: bool = False, noalloc: bool = False,
+               use_graph: bool = False) -> dict:
         """Run one selection; device_inputs=(X_ptr, y_ptr, n, d) skips the upload.
         noalloc (device inputs only): snn_select_device_noalloc, whose device
-        work memory is the library's fixed array."""
+        work memory is the library's fixed array.  use_graph: steps replayed
+        from hipGraphs kept in the context (no phase times)."""
         kv = np.ascontiguousarray(k_vals, dtype=np.int32)
         lc = np.ascontiguousarray(loss_codes, dtype=np.int32)
         if device_inputs is None:
@@ -163,7 +165,8 @@ class Engine:
             xp, yp, n, d = device_inputs
         p = SnnParams(method=int(method), model_type=int(model_type), k_fold=int(k_fold),
                       num_features=int(num_features), auto_feature=int(bool(auto_feature)),
-                      form=int(form), use_graph=0, compensated=int(bool(compensated)))
+                      form=int(form), use_graph=int(bool(use_graph)),
+                      compensated=int(bool(compensated)))
         sel = np.zeros(d, np.int32)
         order = np.zeros(d, np.int32)
         errs = np.zeros(len(lc) * d, np.float32)

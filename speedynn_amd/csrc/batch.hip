@@ -38,6 +38,7 @@ struct Slot {
   bool sorted;       // forward, precision form: sorted columns for the first step
   size_t sort_bytes;
   bool comp;         // keep D's compensation term
+  bool fused;        // precision form, n % 4 == 0: the update writes the next step's seeds
   bool on_device;    // the caller's X and y are device pointers
   const float* dX;   // X and y on the device (the caller's, or copies in the arena)
   const float* dy;
@@ -113,10 +114,13 @@ struct Plan {
     auto* probs = at<snn::SearchProblem>(Mt(), probs_off);
     auto* counter = at<unsigned long long>(Mt(), counter_off);
     auto pruned_group = [&](const Group& g) { return pruned && !(g.mult > 0 && first_step); };
+    // seeds: the previous step's fused update wrote them, except before a
+    // backward problem's first step and for problems without the fused update
     for (const Group& g : groups) {
       if (!pruned_group(g)) continue;
       for (int i = g.first; i < g.first + g.count; ++i) {
         const Slot& sl = slots[i];
+        if (sl.fused && !(first_step && g.mult < 0)) continue;
         HIPCHK(snn::launch_seed_scan(at<float>(A(), sl.D), at<SelState>(A(), sl.state),
                                      at<int>(A(), sl.seeds), sl.pb.n, sl.pb.npf, s));
       }
@@ -150,8 +154,15 @@ struct Plan {
                                 at<int>(A(), sl.ordering), at<float>(A(), sl.errors),
                                 at<float>(A(), sl.val), at<int>(A(), sl.bestk), nullptr,
                                 at<int>(A(), sl.perm), s));
-      HIPCHK(snn::launch_update(at<float>(A(), sl.D), sl.comp ? at<float>(A(), sl.Dlo) : nullptr,
-                                at<float>(A(), sl.XT), pb.n, st, -1, pb.method == 1 ? 1 : -1, s));
+      const int mult = pb.method == 1 ? 1 : -1;
+      float* Dlo = sl.comp ? at<float>(A(), sl.Dlo) : nullptr;
+      if (sl.fused)
+        HIPCHK(snn::launch_update_seeds(at<float>(A(), sl.D), Dlo, at<float>(A(), sl.XT), pb.n,
+                                        st, mult, at<int>(A(), sl.seeds), pb.npf, snn::RowList{},
+                                        nullptr, nullptr, pb.d, nullptr, nullptr, nullptr, s));
+      else
+        HIPCHK(snn::launch_update(at<float>(A(), sl.D), Dlo, at<float>(A(), sl.XT), pb.n, st, -1,
+                                  mult, s));
     }
     return SNN_OK;
   }
@@ -271,6 +282,7 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     s.perm = take(off, sizeof(int) * d);
     s.key = take(off, sizeof(double) * d);
     s.seeds = pruned ? take(off, sizeof(int) * n * 128) : 0;
+    s.fused = pruned && pb.n % 4 == 0;
     const snn_problem& q = probs[s.index];
     s.has_med = snn::needs_median_scratch(q.loss_functions, q.num_loss);
     s.loss_paths = snn::loss_paths(q.loss_functions, q.num_loss);

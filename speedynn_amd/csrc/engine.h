@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -78,6 +79,22 @@ struct EventPool {
   bool ready = false;
 };
 
+// Step graphs of single selections: the launches of one step, captured once
+// per distinct launch sequence (key: every value the launches embed) and
+// replayed; `used` is the pruned-kernel bit mask of the captured step.
+struct StepGraphs {
+  struct Entry {
+    hipGraphExec_t exec = nullptr;
+    int used = 0;
+  };
+  std::map<std::vector<size_t>, Entry> map;
+  void clear() {
+    for (auto& kv : map)
+      if (kv.second.exec) hipGraphExecDestroy(kv.second.exec);
+    map.clear();
+  }
+};
+
 struct snn_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -86,5 +103,6 @@ struct snn_ctx {
       rank, tauw, iperm, frange, lidx, lval, llb, lcount, M16, long_d, long_i, long_lab;
   snn::BatchCache batch;
   EventPool events;
+  StepGraphs step_graphs;
   void* fixed_scratch = nullptr;  // device address of the fixed work array on this device
 };

@@ -234,53 +234,70 @@ hipError_t launch_variant(const snn::SearchProblem& sl, int variant, const int* 
   return snn::launch_search_pruned(sl, variant, s);
 }
 
-int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int* rank,
-                        hipStream_t s, int* used, hipEvent_t* tune_ev) {
+// The pruned kernel a step runs: `variant` (0: the rule by direction), or
+// tuning trials (`tune`) where the shape and step range have no recorded
+// choice and events exist to time them.
+struct PrunedPlan {
+  int variant = 0;
+  bool tune = false;
+  int cands[4] = {1, 0, 0, 0};
+  int nc = 1;
+  int bucket = 0;
+  TuneKey key;
+};
+
+PrunedPlan plan_pruned(const snn::SearchProblem& sl, int d, int step, const int* rank,
+                       bool events) {
   static const bool off = [] {
     const char* e = std::getenv("SPEEDYNN_TUNING");
     return e && !std::strcmp(e, "off");
   }();
-  int bucket = 0;
-  while ((2 << bucket) <= step + 1) ++bucket;
-  if (rank) bucket = 64;
-  const auto key = std::make_tuple(sl.n, d, snn::search_list_size(sl.kmax), sl.mult, bucket);
-  int cands[4] = {1, 0, 0, 0}, nc = 1;  // the kernels that apply to this shape
-  if (snn::pruned_shared_applies(sl)) cands[nc++] = 2;
-  if (snn::pruned_split_applies(sl)) cands[nc++] = 3;
-  if (rank) cands[nc++] = 4;
-  int variant = 0;
-  bool tune = false;
+  PrunedPlan pp;
+  while ((2 << pp.bucket) <= step + 1) ++pp.bucket;
+  if (rank) pp.bucket = 64;
+  pp.key = std::make_tuple(sl.n, d, snn::search_list_size(sl.kmax), sl.mult, pp.bucket);
+  // the kernels that apply to this shape
+  if (snn::pruned_shared_applies(sl)) pp.cands[pp.nc++] = 2;
+  if (snn::pruned_split_applies(sl)) pp.cands[pp.nc++] = 3;
+  if (rank) pp.cands[pp.nc++] = 4;
   // SPEEDYNN_PRUNED=wave|shared|split|walk runs that kernel wherever it
   // applies (read per call, for measurements and the per-kernel tests)
   const char* fe = std::getenv("SPEEDYNN_PRUNED");
   if (fe && *fe) {
     int f = !std::strcmp(fe, "wave") ? 1 : !std::strcmp(fe, "shared") ? 2
           : !std::strcmp(fe, "split") ? 3 : !std::strcmp(fe, "walk") ? 4 : 0;
-    for (int i = 0; i < nc; ++i)
-      if (cands[i] == f) variant = f;
-    if (variant == 0 && f != 4) variant = 1;  // the per-wave kernel applies everywhere
-    HIPCHK(launch_variant(sl, variant, rank, s));
-    const int v = variant ? variant : (sl.mult > 0 || !snn::pruned_shared_applies(sl) ? 1 : 2);
-    *used |= 1 << (v - 1);
-    return SNN_OK;
+    for (int i = 0; i < pp.nc; ++i)
+      if (pp.cands[i] == f) pp.variant = f;
+    if (pp.variant == 0 && f != 4) pp.variant = 1;  // the per-wave kernel applies everywhere
+    return pp;
   }
-  if (!off && nc > 1) {
+  if (!off && pp.nc > 1) {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     load_tuned_locked();
-    auto it = g_tuned.find(key);
+    auto it = g_tuned.find(pp.key);
     if (it != g_tuned.end()) {
-      variant = it->second;
-      if (variant == 4 && !rank) variant = 1;
-    } else if (tune_ev) {
-      tune = true;
+      pp.variant = it->second;
+      if (pp.variant == 4 && !rank) pp.variant = 1;
+    } else if (events) {
+      pp.tune = true;
     }
   }
-  if (!tune) {
+  return pp;
+}
+
+int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int* rank,
+                        hipStream_t s, int* used, hipEvent_t* tune_ev) {
+  PrunedPlan pp = plan_pruned(sl, d, step, rank, tune_ev != nullptr);
+  int variant = pp.variant;
+  if (!pp.tune) {
     HIPCHK(launch_variant(sl, variant, rank, s));
     const int v = variant ? variant : (sl.mult > 0 || !snn::pruned_shared_applies(sl) ? 1 : 2);
     *used |= 1 << (v - 1);
     return SNN_OK;
   }
+  int* cands = pp.cands;
+  const int nc = pp.nc, bucket = pp.bucket;
+  const TuneKey key = pp.key;
   // the kernels' order rotates with the range
   {
     int rot[4];
@@ -601,25 +618,25 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   // precision form: each update also writes the next step's seeds (rows of a
   // multiple of 4 floats); the first pruned step of a backward run scans them
   const bool fused_seeds = pruned && n % 4 == 0;
-  int issued = 0;
-  for (int step = 0; step < pb.max_steps; ++step) {
-    hipEvent_t* mk = nullptr;
-    if (timed) {
-      if (step >= R) {  // the slot's previous step: read its times before reuse
-        rc = retire_step(ctx, step - R, step, nullptr, &pt, trace);
-        if (rc) return rc;
-      }
-      mk = &ep.ring[(size_t)(step % R) * MK];
-    }
-    auto mark = [&](int i) { return timed ? hipEventRecord(mk[i], s) : hipSuccess; };
+  // graph mode (params.use_graph; not with a caller's arena): a step's
+  // launches are captured once per distinct sequence and replayed; a step
+  // whose pruned kernel is still to be tuned runs on the stream.  Graph mode
+  // records no phase times.
+  const bool graphs = prm->use_graph && timed;
+  const bool phases = timed && !graphs;
+  const int lpaths = snn::loss_paths(loss_functions, num_loss);
+
+  // one step's launches; mk: its five phase marks (null: none), tune_ev: the
+  // tuner's events (null: no trials)
+  auto issue = [&](int step, bool refresh, hipEvent_t* mk, hipEvent_t* tune_ev) -> int {
+    auto mark = [&](int i) { return mk ? hipEventRecord(mk[i], s) : hipSuccess; };
     HIPCHK(mark(0));
     HIPCHK(snn::launch_compact(dX, w.cand, st, w.Xc, n, d, pb.cpad, s));
     HIPCHK(mark(1));
     const bool pruned_step = pruned && (step > 0 || !fwd);
-    if (pruned && !fwd && step > 0 && d - step <= m_refresh) {
+    if (refresh) {
       HIPCHK(snn::launch_max_square(w.Xc, st, w.M, n, pb.cpad, s));
       if (w.M16) HIPCHK(snn::launch_round_up_bf16(w.M, w.M16, (int64_t)nn, s));
-      m_refresh = (d - step) / 2;
     }
     // seeds: the previous step's update wrote them where it was fused
     if (pruned_step && !(fused_seeds && step > 0))
@@ -632,8 +649,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
         HIPCHK(snn::launch_rank(w.sidx, st, w.rank, n, s));
         rank = w.rank;
       }
-      int rc2 = search_pruned_tuned(sl, d, step, rank, s, &used_variant,
-                                    timed ? ep.tune : nullptr);
+      int rc2 = search_pruned_tuned(sl, d, step, rank, s, &used_variant, tune_ev);
       if (rc2) return rc2;
     } else if (long_list) {
       HIPCHK(snn::launch_search_long(sl, w.long_d, w.long_i, w.long_lab, long_waves, s));
@@ -641,13 +657,13 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
       // first forward step (D = 0): sorted columns; the dense search runs
       // instead where X is not finite (each kernel checks the state's flag)
       if (sorted_first) HIPCHK(snn::launch_search_sorted(sl, d, s));
-      sl.zero = pruned ? 1 : 0;
-      HIPCHK(snn::launch_search_dense(sl, s));
-      sl.zero = 0;
+      snn::SearchProblem sz = sl;
+      sz.zero = pruned ? 1 : 0;
+      HIPCHK(snn::launch_search_dense(sz, s));
     }
     HIPCHK(mark(3));
     HIPCHK(snn::launch_loss(w.P, dy, w.losses, st, w.err, n, pb.cpad, num_k, num_loss, d, w.med,
-                            snn::loss_paths(loss_functions, num_loss), s));
+                            lpaths, s));
     HIPCHK(snn::launch_select(st, pb, w.kvals, w.err, w.cand, w.sel, w.ordering, w.errors, w.val,
                               w.bestk, w.cand_err, w.perm, s));
     // the forward warm bound's largest square comes from the chosen column
@@ -664,11 +680,85 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
       else
         HIPCHK(snn::launch_update(D, Dlo, w.XT, n, st, -1, fwd ? 1 : -1, s));
     }
+    return SNN_OK;
+  };
+
+  // what every captured launch embeds: buffers, inputs, shapes and options
+  std::vector<size_t> gbase;
+  if (graphs) {
+    const void* ptrs[] = {dX, dy, w.XT, w.D, w.Dlo, w.Xc, w.P, w.err, w.val, w.errors,
+                          w.cand_err, w.bestk, w.cand, w.sel, w.ordering, w.kvals, w.losses,
+                          w.perm, w.iperm, w.st, w.key, w.M, w.skey, w.med, w.tauw, w.frange,
+                          w.lidx, w.lcount, w.lval, w.llb, w.M16, w.seeds, w.sidx, w.rank,
+                          w.visited, w.long_d, w.long_i, w.long_lab};
+    for (const void* q : ptrs) gbase.push_back((size_t)q);
+    const int64_t vals[] = {n, d, pb.npf, pb.cpad, num_k, num_loss, kmax, pb.method,
+                            pb.model_type, pb.auto_feature, pb.max_steps, pb.num_desired,
+                            prm->form, comp, rows_cap, long_waves, lpaths};
+    for (int64_t v : vals) gbase.push_back((size_t)v);
+    for (int l = 0; l < num_loss; ++l) gbase.push_back((size_t)loss_functions[l]);
+    if (ctx->step_graphs.map.size() > 256) ctx->step_graphs.clear();  // bound the cache
+  }
+
+  int issued = 0;
+  for (int step = 0; step < pb.max_steps; ++step) {
+    const bool refresh = pruned && !fwd && step > 0 && d - step <= m_refresh;
+    StepGraphs::Entry* ge = nullptr;
+    if (graphs) {
+      const bool pruned_step = pruned && (step > 0 || !fwd);
+      const int* rank = (sorted_first && step == 1) ? w.rank : nullptr;
+      PrunedPlan pp;
+      if (pruned_step) pp = plan_pruned(sl, d, step, rank, true);
+      if (!pp.tune) {
+        std::vector<size_t> key = gbase;
+        const size_t shape[] = {(size_t)(step == 0), (size_t)refresh, (size_t)(rank != nullptr),
+                                (size_t)(step + 1 < pb.max_steps), (size_t)pp.variant};
+        key.insert(key.end(), shape, shape + 5);
+        auto it = ctx->step_graphs.map.find(key);
+        if (it == ctx->step_graphs.map.end()) {
+          const int before = used_variant;
+          used_variant = 0;
+          HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          const int rc3 = issue(step, refresh, nullptr, nullptr);
+          hipGraph_t g = nullptr;
+          const hipError_t ec = hipStreamEndCapture(s, &g);
+          if (rc3) {
+            if (g) hipGraphDestroy(g);
+            return rc3;
+          }
+          HIPCHK(ec);
+          StepGraphs::Entry en;
+          en.used = used_variant;
+          used_variant = before;
+          const hipError_t ei = hipGraphInstantiate(&en.exec, g, nullptr, nullptr, 0);
+          hipGraphDestroy(g);
+          HIPCHK(ei);
+          it = ctx->step_graphs.map.emplace(std::move(key), en).first;
+        }
+        ge = &it->second;
+      }
+    }
+    if (ge) {
+      HIPCHK(hipGraphLaunch(ge->exec, s));
+      used_variant |= ge->used;
+    } else {
+      hipEvent_t* mk = nullptr;
+      if (phases) {
+        if (step >= R) {  // the slot's previous step: read its times before reuse
+          rc = retire_step(ctx, step - R, step, nullptr, &pt, trace);
+          if (rc) return rc;
+        }
+        mk = &ep.ring[(size_t)(step % R) * MK];
+      }
+      rc = issue(step, refresh, mk, timed ? ep.tune : nullptr);
+      if (rc) return rc;
+    }
+    if (refresh) m_refresh = (d - step) / 2;
     issued = step + 1;
   }
   if (timed) HIPCHK(hipEventRecord(e2, s));
   HIPCHK(hipStreamSynchronize(s));
-  if (timed) {
+  if (phases) {
     for (int t = issued > R ? issued - R : 0; t < issued; ++t) {
       rc = retire_step(ctx, t, issued, e2, &pt, trace);
       if (rc) return rc;
@@ -822,9 +912,11 @@ void snn_destroy(snn_ctx* c) {
                     &c->kvals, &c->losses, &c->state, &c->scratch, &c->seeds,
                     &c->visited, &c->perm, &c->key, &c->M, &c->med, &c->skey,
                     &c->sidx, &c->sort_tmp, &c->rank, &c->tauw, &c->iperm, &c->frange,
-                    &c->lidx, &c->lval, &c->llb, &c->lcount, &c->M16};
+                    &c->lidx, &c->lval, &c->llb, &c->lcount, &c->M16, &c->long_d,
+                    &c->long_i, &c->long_lab};
   for (DevBuf* b : bufs) b->release();
   destroy_events(c);
+  c->step_graphs.clear();
   c->batch.drop_graphs();
   c->batch.arena.release();
   c->batch.meta.release();
