@@ -1301,11 +1301,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_dense(
 }
 
 // Pruned items of a problem: (candidate block, query), candidate-block major;
-// one wave per item, no barriers.
+// one wave per item, no barriers; the register budget of k_search_pruned.
 template <int K, int MULT>
-__global__ __launch_bounds__(256) void k_batch_pruned(const SearchProblem* __restrict__ probs,
-                                                      const int64_t* __restrict__ start, int P,
-                                                      unsigned long long* __restrict__ counter) {
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(K <= 20 && MULT > 0 ? SNN_PRUNED_WAVES(K) : 1)))
+void k_batch_pruned(const SearchProblem* __restrict__ probs, const int64_t* __restrict__ start,
+                    int P, unsigned long long* __restrict__ counter) {
   __shared__ float lds[4 * wave_lds_words<K>()];
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
