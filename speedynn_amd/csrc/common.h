@@ -52,14 +52,18 @@ struct Problem {
 // distance update that precedes the search: for row q the entries
 // e < count[q] of idx/val[q * cap + e] are the eligible points j (outside q's
 // fold), ascending, whose lower bound (forward D_qj; backward fl(D_qj - M_qj),
-// kept in lb) is at most the row's largest warm bound; val holds D_qj.
-// count[q] = -1: no list (no finite warm bound, or more than cap points),
-// scan the row.
+// kept in lb) is at most bound[q]; val holds D_qj.  bound[q] is the largest
+// warm bound of the candidate blocks the list serves (leading blocks whose
+// bound is far above the rest's are left out, so that one loose block does
+// not make every list too long); a search wave whose bound exceeds bound[q]
+// scans the row instead.  count[q] = -1: no list (no finite warm bound, or
+// more than cap points), scan the row.
 struct RowList {
   int* idx;
   float* val;
   float* lb;  // backward only (null forward: the lower bound is val)
   int* count;
+  float* bound;
   int64_t cap;
 };
 

@@ -350,6 +350,7 @@ struct Work {
   double* key = nullptr;
   float *M = nullptr, *skey = nullptr, *med = nullptr, *tauw = nullptr, *frange = nullptr;
   int *lidx = nullptr, *lcount = nullptr;
+  float* lbound = nullptr;
   float *lval = nullptr, *llb = nullptr;
   unsigned short* M16 = nullptr;
   int *seeds = nullptr, *sidx = nullptr, *rank = nullptr;
@@ -513,6 +514,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
       bd.add(ctx->M16, sizeof(unsigned short) * nn, &w.M16);
     }
     bd.add(ctx->lcount, sizeof(int) * (size_t)n, &w.lcount);
+    bd.add(ctx->lbound, sizeof(float) * (size_t)n, &w.lbound);
   }
   if (sorted_first) {
     bd.add(ctx->skey, sizeof(float) * (size_t)n * d, &w.skey);
@@ -603,7 +605,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     sl.iperm = w.iperm;
     sl.XT = w.XT;
     sl.frange = w.frange;
-    if (rows_cap) sl.rows = snn::RowList{w.lidx, w.lval, w.llb, w.lcount, rows_cap};
+    if (rows_cap) sl.rows = snn::RowList{w.lidx, w.lval, w.llb, w.lcount, w.lbound, rows_cap};
   }
 
   // per step: [a] gather [b] bound seeds [b2] search [c] losses + arg-min
@@ -689,7 +691,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     const void* ptrs[] = {dX, dy, w.XT, w.D, w.Dlo, w.Xc, w.P, w.err, w.val, w.errors,
                           w.cand_err, w.bestk, w.cand, w.sel, w.ordering, w.kvals, w.losses,
                           w.perm, w.iperm, w.st, w.key, w.M, w.skey, w.med, w.tauw, w.frange,
-                          w.lidx, w.lcount, w.lval, w.llb, w.M16, w.seeds, w.sidx, w.rank,
+                          w.lidx, w.lcount, w.lbound, w.lval, w.llb, w.M16, w.seeds, w.sidx, w.rank,
                           w.visited, w.long_d, w.long_i, w.long_lab};
     for (const void* q : ptrs) gbase.push_back((size_t)q);
     const int64_t vals[] = {n, d, pb.npf, pb.cpad, num_k, num_loss, kmax, pb.method,
@@ -913,7 +915,7 @@ void snn_destroy(snn_ctx* c) {
                     &c->visited, &c->perm, &c->key, &c->M, &c->med, &c->skey,
                     &c->sidx, &c->sort_tmp, &c->rank, &c->tauw, &c->iperm, &c->frange,
                     &c->lidx, &c->lval, &c->llb, &c->lcount, &c->M16, &c->long_d,
-                    &c->long_i, &c->long_lab};
+                    &c->long_i, &c->long_lab, &c->lbound};
   for (DevBuf* b : bufs) b->release();
   destroy_events(c);
   c->step_graphs.clear();

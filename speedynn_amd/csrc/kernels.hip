@@ -370,20 +370,80 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
     const int64_t fhi = flo + npf < n ? flo + npf : n;
     SeedPair sp;
     sp.reset();
-    // the row list's bound
+    // the row list's bound: the largest warm bound over the candidate blocks
+    // it serves.  Leading blocks (the largest-scale candidates come first)
+    // whose largest bound is more than twice that of every later block are
+    // left out, and their search waves scan the row: one loose block would
+    // otherwise make every list too long for the others.
     float Tq = -kInf;  // -inf: no list
     if (rl.idx && !st->nonfinite) {
-      float m = 0.0f;
-      bool unknown = false;
       const int nc = st->ncand;
-      for (int c = lane; c < nc; c += 64) {
-        const float t = tauw[i * d + iperm[cand[c]]];
-        unknown |= !(t < kMaxF);
-        m = fmaxf(m, t);
-      }
+      const int nb = (nc + 63) / 64;
+      // lane b: block b's largest warm bound and whether one is unknown
+      float mb = 0.0f;
+      bool ub = false;
+      if (nb <= 16) {
+        // four lanes per block, 16 candidates each, then two shuffles
+        const int b = lane >> 2;
+        float t = 0.0f;
+        int unk = 0;
+        if (b < nb) {
+          // the three dependent loads of all 16 candidates in flight together
+          int cs[16];
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-      if (!__any(unknown)) {
+          for (int k = 0; k < 16; ++k) {
+            const int c = b * 64 + (lane & 3) + 4 * k;
+            cs[k] = c < nc ? cand[c] : -1;
+          }
+#pragma unroll
+          for (int k = 0; k < 16; ++k) cs[k] = cs[k] >= 0 ? iperm[cs[k]] : -1;
+#pragma unroll
+          for (int k = 0; k < 16; ++k) {
+            if (cs[k] >= 0) {
+              const float v = tauw[i * d + cs[k]];
+              if (v < kMaxF) t = fmaxf(t, v);
+              else unk = 1;
+            }
+          }
+        }
+        t = fmaxf(t, __shfl_xor(t, 1));
+        t = fmaxf(t, __shfl_xor(t, 2));
+        unk |= __shfl_xor(unk, 1);
+        unk |= __shfl_xor(unk, 2);
+        mb = __shfl(t, (4 * lane) & 63);
+        ub = __shfl(unk, (4 * lane) & 63) != 0;
+      } else
+      for (int b = 0; b < nb; ++b) {
+        const int c = b * 64 + lane;
+        float t = 0.0f;
+        bool unk = false;
+        if (c < nc) {
+          t = tauw[i * d + iperm[cand[c]]];
+          unk = !(t < kMaxF);
+          if (unk) t = 0.0f;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) t = fmaxf(t, __shfl_xor(t, o));
+        const bool bu = __any(unk);
+        if ((b & 63) == lane) {
+          mb = fmaxf(mb, t);
+          ub = ub || bu;
+        }
+      }
+      // suffix maxima over blocks b' >= b (lane b), then the first block kept
+      float sfx = lane < nb ? mb : 0.0f;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const float v = __shfl_down(sfx, o);
+        if (lane + o < 64) sfx = fmaxf(sfx, v);
+      }
+      const float after = __shfl_down(sfx, 1);  // max over blocks > b
+      const bool drop = nb <= 64 && lane + 1 < nb && (ub || mb > 2.0f * after);
+      const unsigned long long keep = __ballot(!drop);  // lanes >= nb count as kept
+      const int b0 = keep ? (int)__builtin_ctzll(keep) : 0;
+      const float m = __shfl(sfx, b0);
+      const bool unknown = __any(ub && lane >= b0 && lane < nb) || nb > 64;
+      if (!unknown) {
         if (mult < 0) {
           Tq = m;
         } else {
@@ -487,7 +547,10 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
       }
     }
     sp.write_ranked(seeds + i * 128);
-    if (rl.idx && lane == 0) rl.count[i] = Tq == -kInf ? -1 : (int)lcount;
+    if (rl.idx && lane == 0) {
+      rl.count[i] = Tq == -kInf ? -1 : (int)lcount;
+      rl.bound[i] = Tq;
+    }
   }
 }
 

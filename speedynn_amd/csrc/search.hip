@@ -590,6 +590,9 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
   const float kInf = __int_as_float(0x7f800000);
   auto dist = [](float dq, float p) { return MULT > 0 ? dq + p : dq - p; };
 
+  // the row list's count and bound, loaded before the seed bound needs the wave
+  const int lcnt0 = a.rows.idx ? a.rows.count[q] : -1;
+  const float lbound = a.rows.idx ? a.rows.bound[q] : -kInf;
   const int rk = cand_rank(a, c, live);
   float thr = seed_bound<K, MULT, PIPE>(a, Xl, Dq, xq, q, cpad, live,
                                         live ? warm_bound<MULT>(a, q, rk) : kInf);
@@ -713,7 +716,8 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
     rtail += (int)__popcll(mask);
     while (rtail - rhead >= BATCH) eval8();
   };
-  const int lcnt = a.rows.idx ? a.rows.count[q] : -1;
+  // the row's list serves this wave if its bound covers the wave's
+  const int lcnt = T > lbound ? -1 : lcnt0;
   if (lcnt >= 0) {
     // the row's list (RowList): its entries ascend in point index, lie
     // outside q's fold and hold every point whose lower bound is <= T; the
