@@ -43,8 +43,8 @@ __device__ __forceinline__ void topk_insert(float (&Ld)[K], int (&Li)[K], float 
   }
 }
 
-// Prediction for the first k neighbours whose labels sit in LDS column
-// `lab` (stride 64 floats per slot).  Regression: nearest-first float sum / k
+// Prediction for the first k neighbours whose labels sit in the column
+// `lab` (stride 64 floats per slot: LDS, or device memory in k_search_long).  Regression: nearest-first float sum / k
 // (speedynn/lib/cpu.c:56-66).  Classification: get_mode_array
 // (speedynn/lib/util.c:165-188): a label's count goes to its first
 // occurrence, and the first position with the strictly largest count wins.
@@ -73,12 +73,55 @@ __device__ __forceinline__ float predict(const float* lab, int k, int model_type
   return best_v;
 }
 
-// Epilogue of one (query, candidate) lane: the list's labels go to the
-// wave's LDS column `lab`, then the prediction of every k value is written to
-// P[m][c][q].
+// Classification vote over the first k of the K neighbours' labels lv
+// (k <= K), held in registers: the rule of predict(), every comparison on
+// registers instead of LDS reads (lists up to 24, where the K x K compare
+// network fits the kernels' register budgets).
+template <int K>
+__device__ __forceinline__ float vote_regs(const float (&lv)[K], int k) {
+  float best_v = lv[0];
+  int best_cnt = 0;
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    if (i < k) {
+      const float v = lv[i];
+      bool first = true;
+      int cnt = 0;
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        if (j < k) {
+          const bool eq = (j == i) || (lv[j] == v);
+          cnt += eq ? 1 : 0;
+          if (j < i && eq) first = false;
+        }
+      }
+      if (first && cnt > best_cnt) {
+        best_cnt = cnt;
+        best_v = v;
+      }
+    }
+  }
+  return best_v;
+}
+
+// Epilogue of one (query, candidate) lane: the prediction of every k value,
+// written to P[m][c][q].  Classification with lists up to 24 votes on
+// registers; otherwise the labels go to the wave's LDS column `lab`.
 template <int K>
 __device__ __forceinline__ void finish_query(const SearchProblem& a, const int (&Li)[K],
                                              float* lab, int64_t q, int c, bool live) {
+  if constexpr (K <= 24) {
+    if (a.model_type == 1) {
+      float lv[K];
+#pragma unroll
+      for (int m = 0; m < K; ++m) lv[m] = a.y[Li[m]];
+      for (int m = 0; m < a.nk; ++m) {
+        const float pred = vote_regs<K>(lv, a.kvals[m]);
+        if (live) a.P[((int64_t)m * a.cpad + c) * a.n + q] = pred;
+      }
+      return;
+    }
+  }
 #pragma unroll
   for (int m = 0; m < K; ++m) lab[m * 64] = a.y[Li[m]];
   for (int m = 0; m < a.nk; ++m) {
