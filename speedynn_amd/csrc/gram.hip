@@ -167,21 +167,18 @@ __global__ __launch_bounds__(64 * TB * TB) void k_gram_mfma(const bf16x8* __rest
         fa[u][p] = lds[buf][(fa0 + u * 3 + p) * 64 + lane];
         fb[u][p] = lds[buf][(fb0 + u * 3 + p) * 64 + lane];
       }
+    // six split products per accumulator, small terms first (hi.lo, lo.hi,
+    // mid.mid, hi.mid, mid.hi, hi.hi); the four accumulators take turns, so
+    // consecutive MFMAs never wait on each other's result
+    constexpr int PA[6] = {0, 2, 1, 0, 1, 0}, PB[6] = {2, 0, 1, 1, 0, 0};
 #pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
+    for (int pr = 0; pr < 6; ++pr)
 #pragma unroll
-      for (int tj = 0; tj < 2; ++tj) {
-        const bf16x8* x = fa[ti];
-        const bf16x8* y = fb[tj];
-        floatx16 c = acc[ti][tj];  // small terms first
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[0], y[2], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[2], y[0], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[1], y[1], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[0], y[1], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[1], y[0], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[0], y[0], c, 0, 0, 0);
-        acc[ti][tj] = c;
-      }
+      for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+          acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[ti][PA[pr]], fb[tj][PB[pr]],
+                                                                acc[ti][tj], 0, 0, 0);
     if (next) put(buf ^ 1);  // the other buffer was last read before the previous barrier
     __syncthreads();
   }
