@@ -107,6 +107,11 @@ struct SearchProblem {
   const float* frange;
   int d;
   RowList rows;  // short lists instead of D (and M) rows where count >= 0 (idx null: none)
+  // Candidate blocks a single-problem launch covers: those holding live
+  // candidates this step (every step moves exactly one feature, so step s of
+  // a selection over d features has d - s); 0: all cpad / 64.  Launching only
+  // these keeps workgroups from holding LDS for waves with nothing to do.
+  int nblk;
 };
 
 // Counters of the point evaluations a pruned search performs: one counter
@@ -119,6 +124,10 @@ constexpr int kVisitStride = 16;  // unsigned long longs per counter: 128 bytes
 __device__ __forceinline__ void count_visits(unsigned long long* visited, unsigned int wave_id,
                                              unsigned long long v) {
   atomicAdd(visited + (size_t)(wave_id & (kVisitSlots - 1)) * kVisitStride, v);
+}
+
+__host__ __device__ inline int live_blocks(const SearchProblem& a) {
+  return a.nblk > 0 ? a.nblk : a.cpad / 64;
 }
 
 // Queries a pruned-search launch covers, and the point of the i-th.

@@ -630,8 +630,15 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
 
   // one step's launches; mk: its five phase marks (null: none), tune_ev: the
   // tuner's events (null: no trials)
+  // candidate blocks with live candidates at a step (each step moves one
+  // feature out of the candidate list, or the selection has stopped)
+  auto blocks_at = [&](int step) {
+    const int live = (d - step + 63) / 64, all = pb.cpad / 64;
+    return live < 1 ? 1 : (live < all ? live : all);
+  };
   auto issue = [&](int step, bool refresh, hipEvent_t* mk, hipEvent_t* tune_ev) -> int {
     auto mark = [&](int i) { return mk ? hipEventRecord(mk[i], s) : hipSuccess; };
+    sl.nblk = blocks_at(step);
     HIPCHK(mark(0));
     HIPCHK(snn::launch_compact(dX, w.cand, st, w.Xc, n, d, pb.cpad, s));
     HIPCHK(mark(1));
@@ -710,12 +717,14 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
       const bool pruned_step = pruned && (step > 0 || !fwd);
       const int* rank = (sorted_first && step == 1) ? w.rank : nullptr;
       PrunedPlan pp;
+      sl.nblk = blocks_at(step);
       if (pruned_step) pp = plan_pruned(sl, d, step, rank, true);
       if (!pp.tune) {
         std::vector<size_t> key = gbase;
         const size_t shape[] = {(size_t)(step == 0), (size_t)refresh, (size_t)(rank != nullptr),
-                                (size_t)(step + 1 < pb.max_steps), (size_t)pp.variant};
-        key.insert(key.end(), shape, shape + 5);
+                                (size_t)(step + 1 < pb.max_steps), (size_t)pp.variant,
+                                (size_t)sl.nblk};
+        key.insert(key.end(), shape, shape + 6);
         auto it = ctx->step_graphs.map.find(key);
         if (it == ctx->step_graphs.map.end()) {
           const int before = used_variant;

@@ -101,7 +101,7 @@ hipError_t launch_init_state(SelState* st, const Problem& pb, int* cand, int* se
 int64_t batch_dense_items(const SearchProblem& a);
 int64_t batch_pruned_items(const SearchProblem& a);
 // Seeds a backward pruned search reads for its bound (the nearest in D first):
-// 2K + 16 rounded up to 8, at most 128; SPEEDYNN_NSEED overrides it.
+// K + 16 rounded up to 8, at most 128; SPEEDYNN_NSEED overrides it.
 int backward_seed_count(int kmax);
 int batch_grid(int mode, int mult, int K);
 hipError_t launch_batch_search(const SearchProblem* probs, const int64_t* start, int P,

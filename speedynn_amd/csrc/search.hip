@@ -918,7 +918,7 @@ __attribute__((amdgpu_waves_per_eu(K <= 20 && MULT > 0 ? SNN_PRUNED_WAVES(K) : 1
 void k_search_pruned(const SearchProblem a, int cb_major) {
   __shared__ float lds[4 * wave_lds_words<K>()];  // pending buffers, then epilogue labels
   const int wave = uniform(threadIdx.x >> 6);
-  const int nblk = a.cpad / 64;
+  const int nblk = live_blocks(a);
   const int64_t nq = launch_queries(a);
   const int64_t per_xcd = (nq + 7) / 8;
   const int64_t local = (int64_t)(blockIdx.x / 8) * 4 + wave;
@@ -970,7 +970,7 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
   // of each half of the run stay within half of Xc's columns
   const int cb = wave + W * blockIdx.y;
   if ((int)blockIdx.y * W * 64 >= ncand) return;  // no live block in this part: whole workgroup
-  const bool active = cb * 64 < ncand && cb < a.cpad / 64;
+  const bool active = cb * 64 < ncand && cb < live_blocks(a);
   const int c = cb * 64 + lane;
   const bool live = active && c < ncand;
   const int cpad = a.cpad;
@@ -1155,7 +1155,7 @@ __global__ __launch_bounds__(256) void k_search_walk(const SearchProblem a,
   __shared__ float lds[4 * wave_lds_words<K>()];
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int nblk = a.cpad / 64;
+  const int nblk = live_blocks(a);
   const int64_t nq = launch_queries(a);
   const int64_t per_xcd = (nq + 7) / 8;
   const int64_t local = (int64_t)(blockIdx.x / 8) * 4 + wave;
@@ -1491,7 +1491,7 @@ hipError_t launch_search_long(const SearchProblem& a, float* ld, int* li, float*
 template <int K, int QW, int WAVES>
 static hipError_t launch_dense_k(const SearchProblem& a, hipStream_t s) {
   const int qpb = QW * WAVES;
-  dim3 g((unsigned)((a.n + qpb - 1) / qpb), (unsigned)(a.cpad / 64));
+  dim3 g((unsigned)((a.n + qpb - 1) / qpb), (unsigned)live_blocks(a));
   dim3 b(64 * WAVES);
   if (a.mult < 0)
     hipLaunchKernelGGL((k_search_dense<K, QW, WAVES, -1, false>), g, b, 0, s, a);
@@ -1517,14 +1517,14 @@ int backward_seed_count(int kmax) {
 template <int K, int W>
 static void launch_pruned_shared(const SearchProblem& a, hipStream_t s) {
   // workgroups per query: enough W-wave groups to cover the candidate blocks
-  const dim3 g((unsigned)launch_queries(a), (unsigned)((a.cpad / 64 + W - 1) / W)), b(64 * W);
+  const dim3 g((unsigned)launch_queries(a), (unsigned)((live_blocks(a) + W - 1) / W)), b(64 * W);
   if (a.mult > 0) hipLaunchKernelGGL((k_search_pruned_shared<K, 1, W>), g, b, 0, s, a);
   else hipLaunchKernelGGL((k_search_pruned_shared<K, -1, W>), g, b, 0, s, a);
 }
 
 template <int K>
 static hipError_t launch_pruned_k(const SearchProblem& a, int variant, hipStream_t s) {
-  const int nblk = a.cpad / 64;
+  const int nblk = live_blocks(a);
   // the shared-scan kernel: up to 16 candidate blocks (16 waves need lists of
   // at most 20 to stay within 128 registers); its label area needs K <= 50
   const int W = nblk <= 4 ? 4 : nblk <= 8 ? 8 : nblk <= 16 && K <= 20 ? 16 : 0;
@@ -1606,14 +1606,14 @@ hipError_t launch_search_pruned(const SearchProblem& a, int variant, hipStream_t
 }
 
 bool pruned_shared_applies(const SearchProblem& a) {
-  const int K = search_list_size(a.kmax), nblk = a.cpad / 64;
+  const int K = search_list_size(a.kmax), nblk = live_blocks(a);
   return K > 0 && K <= 50 && (nblk <= 8 || (nblk <= 16 && K <= 20));
 }
 
 template <int K>
 static hipError_t launch_walk_k(const SearchProblem& a, const int* rank, hipStream_t s) {
   const int64_t per_xcd = (launch_queries(a) + 7) / 8;
-  const int64_t slice_blocks = (per_xcd * (a.cpad / 64) + 3) / 4;
+  const int64_t slice_blocks = (per_xcd * live_blocks(a) + 3) / 4;
   hipLaunchKernelGGL(k_search_walk<K>, dim3((unsigned)(8 * slice_blocks)), dim3(256), 0, s, a,
                      rank);
   return hipGetLastError();
@@ -1637,7 +1637,7 @@ hipError_t launch_rank(const int* sidx, const SelState* st, int* rank, int64_t n
 }
 
 bool pruned_split_applies(const SearchProblem& a) {
-  const int K = search_list_size(a.kmax), nblk = a.cpad / 64;
+  const int K = search_list_size(a.kmax), nblk = live_blocks(a);
   return K > 0 && K <= 50 && nblk > 4 && nblk <= 16;
 }
 
