@@ -29,7 +29,8 @@ struct SelState {
   int pending;    // fstar chosen but not yet applied to D
   int fslot;      // candidate slot of fstar in the step that chose it
   int nonfinite;  // X holds NaN or infinity (the sorted first step does not apply)
-  int pad;
+  int fdefer;     // feature whose term D in memory still holds (deferred write; the
+                  // search's SearchProblem::defer says whether it applies)
 };
 
 // Fixed description of one selection problem on the device.
@@ -112,6 +113,12 @@ struct SearchProblem {
   // a selection over d features has d - s); 0: all cpad / 64.  Launching only
   // these keeps workgroups from holding LDS for waves with nothing to do.
   int nblk;
+  // Backward pruned searches: D in memory still holds the term of feature
+  // st->fdefer (the update before this search deferred its write); every D
+  // value read from the rows is first reduced by it, fl(D - fl(t^2)) with
+  // t = x_q - x_j of that feature, as the update would have written it.  The
+  // row lists hold the reduced values already.
+  int defer;
 };
 
 // Counters of the point evaluations a pruned search performs: one counter

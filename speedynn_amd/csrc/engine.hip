@@ -630,6 +630,18 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
 
   // one step's launches; mk: its five phase marks (null: none), tune_ev: the
   // tuner's events (null: no trials)
+  // deferred writes of D (backward precision form with row lists): the
+  // update after an even step leaves its term out of D in memory, the next
+  // search's row reads subtract it, and the update after it applies both
+  // terms and writes D -- every other n^2 write saved, every value the same.
+  // SPEEDYNN_DEFER=off writes D after every step.
+  static const bool defer_off = [] {
+    const char* e = std::getenv("SPEEDYNN_DEFER");
+    return e && !std::strcmp(e, "off");
+  }();
+  const bool defer_on = pruned && !fwd && rows_cap > 0 && !comp && !defer_off;
+  auto search_defers = [&](int step) { return defer_on && step % 2 == 1 ? 1 : 0; };
+  auto update_mode = [&](int step) { return defer_on ? (step % 2 == 0 ? 1 : 2) : 0; };
   // candidate blocks with live candidates at a step (each step moves one
   // feature out of the candidate list, or the selection has stopped)
   auto blocks_at = [&](int step) {
@@ -639,6 +651,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   auto issue = [&](int step, bool refresh, hipEvent_t* mk, hipEvent_t* tune_ev) -> int {
     auto mark = [&](int i) { return mk ? hipEventRecord(mk[i], s) : hipSuccess; };
     sl.nblk = blocks_at(step);
+    sl.defer = search_defers(step);
     HIPCHK(mark(0));
     HIPCHK(snn::launch_compact(dX, w.cand, st, w.Xc, n, d, pb.cpad, s));
     HIPCHK(mark(1));
@@ -685,7 +698,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
       if (fused_seeds)
         HIPCHK(snn::launch_update_seeds(D, Dlo, w.XT, n, st, fwd ? 1 : -1, w.seeds, pb.npf,
                                         sl.rows, w.tauw, w.frange, d, w.cand, w.iperm, w.M16,
-                                        s));
+                                        s, update_mode(step)));
       else
         HIPCHK(snn::launch_update(D, Dlo, w.XT, n, st, -1, fwd ? 1 : -1, s));
     }
@@ -718,13 +731,14 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
       const int* rank = (sorted_first && step == 1) ? w.rank : nullptr;
       PrunedPlan pp;
       sl.nblk = blocks_at(step);
+      sl.defer = search_defers(step);
       if (pruned_step) pp = plan_pruned(sl, d, step, rank, true);
       if (!pp.tune) {
         std::vector<size_t> key = gbase;
         const size_t shape[] = {(size_t)(step == 0), (size_t)refresh, (size_t)(rank != nullptr),
                                 (size_t)(step + 1 < pb.max_steps), (size_t)pp.variant,
-                                (size_t)sl.nblk};
-        key.insert(key.end(), shape, shape + 6);
+                                (size_t)sl.nblk, (size_t)sl.defer, (size_t)update_mode(step)};
+        key.insert(key.end(), shape, shape + 8);
         auto it = ctx->step_graphs.map.find(key);
         if (it == ctx->step_graphs.map.end()) {
           const int before = used_variant;

@@ -19,11 +19,13 @@ hipError_t launch_max_square(const float* Xc, const SelState* st, float* M, int6
 // rl.idx set also writes the next search's row lists (RowList) from the warm
 // bounds tauw (forward: and the chosen column's range frange; backward: the
 // lower bounds fl(D - M16) from M16 = M rounded up to bfloat16).
+// defer: 0 write D; 1 leave this step's term out of D (st->fdefer); 2 apply
+// the left-out term and this step's, and write D (see k_update_seeds).
 hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
-                               const SelState* st, int mult, int* seeds, int64_t npf,
+                               SelState* st, int mult, int* seeds, int64_t npf,
                                const RowList& rl, const float* tauw, const float* frange, int d,
                                const int* cand, const int* iperm, const unsigned short* M16,
-                               hipStream_t s);
+                               hipStream_t s, int defer = 0);
 // M16 = M rounded up to bfloat16, count values.
 hipError_t launch_round_up_bf16(const float* M, unsigned short* M16, int64_t count,
                                 hipStream_t s);

@@ -345,11 +345,18 @@ __device__ __forceinline__ v4f bf16x4(const unsigned short* p) {
 // backward lower bound fl(D - M16) uses M rounded up to bfloat16 (M16 >= M),
 // from the rows as they stand; a later refresh only raises the bound, so the
 // list still holds every point the search can visit.
-template <bool COMP>
+//
+// DEF (deferred writes, backward lists): 0 applies fstar and writes D; 1
+// applies it, writes the seeds and lists but not D, and records fstar in
+// st->fdefer (D in memory then lacks that one term, and the next step's
+// readers subtract it themselves, in the same arithmetic); 2 applies the
+// recorded feature first, then fstar, and writes D.  Alternating 1 and 2
+// saves every other write of the n^2 floats, each step's values unchanged.
+template <bool COMP, int DEF>
 __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
                                                       float* __restrict__ Dlo,
                                                       const float* __restrict__ XT, int64_t n,
-                                                      const SelState* __restrict__ st, int mult,
+                                                      SelState* __restrict__ st, int mult,
                                                       int* __restrict__ seeds, int64_t npf,
                                                       RowList rl, const float* __restrict__ tauw,
                                                       const float* __restrict__ frange, int d,
@@ -361,9 +368,13 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
   const int lane = threadIdx.x & 63;
   const float kInf = __int_as_float(0x7f800000);
   const float* __restrict__ xf = XT + (int64_t)f * n;
+  // DEF 2: the feature the previous step left out of D
+  const float* __restrict__ xp = DEF == 2 ? XT + (int64_t)st->fdefer * n : nullptr;
+  if (DEF == 1 && blockIdx.x == 0 && threadIdx.x == 0) st->fdefer = f;  // read by later launches
   for (int64_t i = (int64_t)blockIdx.x * 4 + uniform(threadIdx.x >> 6); i < n;
        i += (int64_t)gridDim.x * 4) {
     const float xi = xf[i];
+    const float xip = DEF == 2 ? xp[i] : 0.0f;
     float* __restrict__ row = D + i * n;
     float* __restrict__ rlo = COMP ? Dlo + i * n : nullptr;
     const int64_t flo = (i / npf) * npf;
@@ -461,11 +472,17 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
     // in: the lane holds four points of the row (all lanes call this together);
     // mv: their M values (backward lists)
     v4f lbv = {0.f, 0.f, 0.f, 0.f};
-    auto upd = [&](v4f& dv, v4f& lv, const v4f xj, const v4f mv, int64_t j, bool in) {
+    auto upd = [&](v4f& dv, v4f& lv, const v4f xj, const v4f mv, const v4f pj, int64_t j,
+                   bool in) {
       int keep = 0;
       if (in) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
+          if (DEF == 2) {  // the deferred term first, as the previous step would have
+            const float t0 = xip - pj[e];
+            const float p0 = t0 * t0;
+            dv[e] = dv[e] + (mult > 0 ? p0 : -p0);
+          }
           const float t = xi - xj[e];
           const float p = t * t;
           const float ps = mult > 0 ? p : -p;
@@ -514,18 +531,21 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
     int64_t jb = 0;
     for (; jb + U * 256 <= n; jb += U * 256) {
       const int64_t j = jb + (int64_t)lane * 4;
-      v4f dv[U], lv[U], xj[U], mv[U];
+      v4f dv[U], lv[U], xj[U], mv[U], pj[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         xj[u] = *reinterpret_cast<const v4f*>(xf + j + u * 256);
         dv[u] = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j + u * 256));
         if (COMP) lv[u] = *reinterpret_cast<v4f*>(rlo + j + u * 256);
         mv[u] = mrow ? bf16x4(mrow + j + u * 256) : v4f{0.f, 0.f, 0.f, 0.f};
+        pj[u] = DEF == 2 ? *reinterpret_cast<const v4f*>(xp + j + u * 256)
+                         : v4f{0.f, 0.f, 0.f, 0.f};
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        upd(dv[u], lv[u], xj[u], mv[u], j + u * 256, true);
-        __builtin_nontemporal_store(dv[u], reinterpret_cast<v4f*>(row + j + u * 256));
+        upd(dv[u], lv[u], xj[u], mv[u], pj[u], j + u * 256, true);
+        if (DEF != 1)
+          __builtin_nontemporal_store(dv[u], reinterpret_cast<v4f*>(row + j + u * 256));
         if (COMP) *reinterpret_cast<v4f*>(rlo + j + u * 256) = lv[u];
       }
     }
@@ -533,16 +553,17 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
       const int64_t j = jb + (int64_t)lane * 4;
       const bool in = j < n;
       v4f lv = {0.f, 0.f, 0.f, 0.f}, xj = {0.f, 0.f, 0.f, 0.f}, dv = {0.f, 0.f, 0.f, 0.f};
-      v4f mv = {0.f, 0.f, 0.f, 0.f};
+      v4f mv = {0.f, 0.f, 0.f, 0.f}, pj = {0.f, 0.f, 0.f, 0.f};
       if (in) {
         xj = *reinterpret_cast<const v4f*>(xf + j);
         dv = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j));
         if (COMP) lv = *reinterpret_cast<v4f*>(rlo + j);
         if (mrow) mv = bf16x4(mrow + j);
+        if (DEF == 2) pj = *reinterpret_cast<const v4f*>(xp + j);
       }
-      upd(dv, lv, xj, mv, j, in);
+      upd(dv, lv, xj, mv, pj, j, in);
       if (in) {
-        __builtin_nontemporal_store(dv, reinterpret_cast<v4f*>(row + j));
+        if (DEF != 1) __builtin_nontemporal_store(dv, reinterpret_cast<v4f*>(row + j));
         if (COMP) *reinterpret_cast<v4f*>(rlo + j) = lv;
       }
     }
@@ -1316,19 +1337,25 @@ hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const
 }
 
 hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
-                               const SelState* st, int mult, int* seeds, int64_t npf,
+                               SelState* st, int mult, int* seeds, int64_t npf,
                                const RowList& rl, const float* tauw, const float* frange, int d,
                                const int* cand, const int* iperm, const unsigned short* M16,
-                               hipStream_t s) {
-  if (n % 4) return hipErrorInvalidValue;
+                               hipStream_t s, int defer) {
+  if (n % 4 || defer < 0 || defer > 2 || (defer && Dlo)) return hipErrorInvalidValue;
   const int64_t rows = (n + 3) / 4;
   const int g = (int)(rows < 8192 ? rows : 8192);
   if (Dlo)
-    hipLaunchKernelGGL((k_update_seeds<true>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st, mult,
-                       seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
+    hipLaunchKernelGGL((k_update_seeds<true, 0>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st,
+                       mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
+  else if (defer == 1)
+    hipLaunchKernelGGL((k_update_seeds<false, 1>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st,
+                       mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
+  else if (defer == 2)
+    hipLaunchKernelGGL((k_update_seeds<false, 2>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st,
+                       mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
   else
-    hipLaunchKernelGGL((k_update_seeds<false>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st, mult,
-                       seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
+    hipLaunchKernelGGL((k_update_seeds<false, 0>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st,
+                       mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
   return hipGetLastError();
 }
 

@@ -5,6 +5,7 @@
 // serve many problems from one work queue.
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "common.h"
 #include "kernels.h"
@@ -420,6 +421,18 @@ __device__ __forceinline__ float seed_bound(const SearchProblem& a, const float*
   // which the seeds rarely improve; where every live lane has one, the 32
   // seed gathers are skipped
   if (MULT < 0 && !__ballot(live && !(warm < kInf))) return live ? warm : -kInf;
+  // a deferred term of D (SearchProblem::defer): subtracted from each seed's D
+  const float* __restrict__ xpr =
+      MULT < 0 && a.defer ? a.XT + (int64_t)a.st->fdefer * a.n : nullptr;
+  const float xqp = xpr ? xpr[q] : 0.0f;
+  auto dval = [&](int jj) {
+    float v = Dq[jj];
+    if (MULT < 0 && xpr) {
+      const float t = xqp - xpr[jj];
+      v = v - t * t;
+    }
+    return v;
+  };
   float B[K];
 #pragma unroll
   for (int m = 0; m < K; ++m) B[m] = kInf;
@@ -434,7 +447,7 @@ __device__ __forceinline__ float seed_bound(const SearchProblem& a, const float*
         js[u] = sq[s0 + u];
         const int jj = js[u] < 0 ? 0 : js[u];
         xs[u] = Xl[(int64_t)jj * cpad];
-        dqs[u] = Dq[jj];
+        dqs[u] = dval(jj);
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -460,7 +473,7 @@ __device__ __forceinline__ float seed_bound(const SearchProblem& a, const float*
       j_[u] = sq[s0 + u];
       const int jj = j_[u] < 0 ? 0 : j_[u];
       x_[u] = Xl[(int64_t)jj * cpad];
-      d_[u] = Dq[jj];
+      d_[u] = dval(jj);
     }
   };
   auto insert = [&](const int (&j_)[8], const float (&x_)[8], const float (&d_)[8]) {
@@ -798,29 +811,50 @@ __device__ __forceinline__ void pruned_wave(const SearchProblem& a, float* buf, 
       }
     }
   } else {
-  float dsl[4], msl[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    dsl[u] = ldD(64 * u + lane);
-    msl[u] = ldM(64 * u + lane);
-  }
-  const int flo32 = (int)flo, fhi32 = (int)fhi;
-  for (int g0 = 0; g0 < n32; g0 += 256) {
+  // the row scan; DF: D in memory still holds a deferred term
+  // (SearchProblem::defer), subtracted from every D value as it arrives
+  auto scan_row = [&](auto df_tag) {
+    constexpr bool DF = MULT < 0 && decltype(df_tag)::value;
+    const float* __restrict__ xpr = DF ? a.XT + (int64_t)a.st->fdefer * n : nullptr;
+    const float xqp = DF ? xpr[q] : 0.0f;
+    auto ldP = [&](int j) { return DF ? xpr[j < n32 ? j : n32 - 1] : 0.0f; };
+    float dsl[4], msl[4], psl[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int jl = g0 + 64 * u + lane;
-      const float lb = MULT > 0 ? dsl[u] : dsl[u] - msl[u];
-      const bool ok = jl < n32 && (jl < flo32 || jl >= fhi32) && lb <= T;
-      const unsigned long long mask = __ballot(ok);
-      if (mask) {
-        seen += __popcll(mask);
-        visit(jl, dsl[u], mask);
-      }
-      // the slot's next chunk, issued once its value is dead, so the load
-      // lands in the slot's own register
-      dsl[u] = ldD(jl + 256);
-      if (MULT < 0) msl[u] = ldM(jl + 256);
+      dsl[u] = ldD(64 * u + lane);
+      msl[u] = ldM(64 * u + lane);
+      psl[u] = ldP(64 * u + lane);
     }
+    const int flo32 = (int)flo, fhi32 = (int)fhi;
+    for (int g0 = 0; g0 < n32; g0 += 256) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int jl = g0 + 64 * u + lane;
+        float dv = dsl[u];
+        if (DF) {
+          const float t = xqp - psl[u];
+          dv = dv - t * t;
+        }
+        const float lb = MULT > 0 ? dv : dv - msl[u];
+        const bool ok = jl < n32 && (jl < flo32 || jl >= fhi32) && lb <= T;
+        const unsigned long long mask = __ballot(ok);
+        if (mask) {
+          seen += __popcll(mask);
+          visit(jl, dv, mask);
+        }
+        // the slot's next chunk, issued once its value is dead, so the load
+        // lands in the slot's own register
+        dsl[u] = ldD(jl + 256);
+        if (MULT < 0) msl[u] = ldM(jl + 256);
+        if (DF) psl[u] = ldP(jl + 256);
+      }
+    }
+  };
+  if constexpr (MULT < 0) {
+    if (a.defer) scan_row(std::true_type{});
+    else scan_row(std::false_type{});
+  } else {
+    scan_row(std::false_type{});
   }
   }
   if (rtail > rhead) eval_tail();
@@ -1012,12 +1046,18 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
   auto ldM = [&](int j) {
     return (MULT < 0 && j < n32) ? __builtin_nontemporal_load(Mq + j) : 0.0f;
   };
+  // a deferred term of D (SearchProblem::defer), subtracted as D arrives
+  const bool df = MULT < 0 && a.defer;
+  const float* __restrict__ xpr = df ? a.XT + (int64_t)a.st->fdefer * n : nullptr;
+  const float xqp = df ? xpr[q] : 0.0f;
+  auto ldP = [&](int j) { return df && j < n32 ? xpr[j] : 0.0f; };
   const int t0 = threadIdx.x;
-  float dn[R], mn[R];  // the next chunk's D (and M), loaded one chunk ahead
+  float dn[R], mn[R], pn[R];  // the next chunk's D (and M), loaded one chunk ahead
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     dn[r] = ldD(r * SUB + t0);
     mn[r] = ldM(r * SUB + t0);
+    pn[r] = ldP(r * SUB + t0);
   }
   const uint32_t ucpad = (uint32_t)cpad;  // n * cpad < 2^31 (checked on the host)
   __syncthreads();
@@ -1032,9 +1072,14 @@ __global__ __launch_bounds__(64 * W) void k_search_pruned_shared(const SearchPro
     for (int r = 0; r < R; ++r) {
       const int j = c0 + r * SUB + t0;
       dv[r] = dn[r];
-      lbv[r] = MULT > 0 ? dn[r] : dn[r] - mn[r];
+      if (df) {
+        const float t = xqp - pn[r];
+        dv[r] = dv[r] - t * t;
+      }
+      lbv[r] = MULT > 0 ? dv[r] : dv[r] - mn[r];
       dn[r] = ldD(j + CH);
       if (MULT < 0) mn[r] = ldM(j + CH);
+      if (df) pn[r] = ldP(j + CH);
       ok[r] = j < n && (uint32_t)(j - flo) >= (uint32_t)fspan && lbv[r] <= Tmax;
       m[r] = __ballot(ok[r]);
       if (lane == 0) wcount[r * W + wave] = (int)__popcll(m[r]);
