@@ -173,6 +173,10 @@ struct SeedPair {
   }
   __device__ __forceinline__ void offer(float v, int64_t j, int64_t flo, int64_t fhi) {
     if (j >= flo && j < fhi) return;
+    offer_eligible(v, j);
+  }
+  // j known to lie outside the fold
+  __device__ __forceinline__ void offer_eligible(float v, int64_t j) {
     if (v < b1) {
       if (v < b0) {
         b1 = b0;

@@ -352,8 +352,13 @@ __device__ __forceinline__ v4f bf16x4(const unsigned short* p) {
 // readers subtract it themselves, in the same arithmetic); 2 applies the
 // recorded feature first, then fstar, and writes D.  Alternating 1 and 2
 // saves every other write of the n^2 floats, each step's values unchanged.
-template <bool COMP, int DEF>
-__global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
+// SEEDS: write the seeds (false for backward lists: every backward search
+// after the first has a warm bound for each candidate and reads no seed, and
+// where one would, the seeds already there -- eligible points, as any K
+// distinct ones bound the K-th distance -- still give a valid bound).
+template <bool COMP, int DEF, bool SEEDS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COMP ? 1 : 4)))
+void k_update_seeds(float* __restrict__ D,
                                                       float* __restrict__ Dlo,
                                                       const float* __restrict__ XT, int64_t n,
                                                       SelState* __restrict__ st, int mult,
@@ -472,8 +477,11 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
     // in: the lane holds four points of the row (all lanes call this together);
     // mv: their M values (backward lists)
     v4f lbv = {0.f, 0.f, 0.f, 0.f};
+    // FOLD (compile time): the lane's points may lie in row i's fold; a block
+    // of points wholly outside it skips the per-point fold tests
     auto upd = [&](v4f& dv, v4f& lv, const v4f xj, const v4f mv, const v4f pj, int64_t j,
-                   bool in) {
+                   bool in, auto fold_tag) {
+      constexpr bool FOLD = decltype(fold_tag)::value;
       int keep = 0;
       if (in) {
 #pragma unroll
@@ -489,12 +497,16 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
           const float s = dv[e] + ps;
           if (COMP) lv[e] += comp_error(xi, xj[e], t, p, dv[e], ps, s, mult < 0);
           dv[e] = s;
-          sp.offer(s, j + e, flo, fhi);
+          if (SEEDS) {
+            if (FOLD) sp.offer(s, j + e, flo, fhi);
+            else sp.offer_eligible(s, j + e);
+          }
           lbv[e] = mrow ? s - mv[e] : s;
-          keep |= (lbv[e] <= Tq && (j + e < flo || j + e >= fhi)) ? 1 << e : 0;
+          keep |= (lbv[e] <= Tq && (!FOLD || j + e < flo || j + e >= fhi)) ? 1 << e : 0;
         }
       }
-      if (Tq != -kInf) {
+      // most blocks of a row add nothing to its list: one ballot then
+      if (Tq != -kInf && __ballot(keep != 0)) {
         // entries in ascending index: lane L's four points precede lane L+1's
         const int c = __builtin_popcount(keep);
         int pre = 0, tot = 0;
@@ -528,8 +540,7 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
     // past the row's end hold no point (n % 4 == 0: a lane's four points are
     // all in the row or all past it)
     constexpr int U = 4;
-    int64_t jb = 0;
-    for (; jb + U * 256 <= n; jb += U * 256) {
+    auto block = [&](int64_t jb, auto fold_tag) {
       const int64_t j = jb + (int64_t)lane * 4;
       v4f dv[U], lv[U], xj[U], mv[U], pj[U];
 #pragma unroll
@@ -543,11 +554,16 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        upd(dv[u], lv[u], xj[u], mv[u], pj[u], j + u * 256, true);
+        upd(dv[u], lv[u], xj[u], mv[u], pj[u], j + u * 256, true, fold_tag);
         if (DEF != 1)
           __builtin_nontemporal_store(dv[u], reinterpret_cast<v4f*>(row + j + u * 256));
         if (COMP) *reinterpret_cast<v4f*>(rlo + j + u * 256) = lv[u];
       }
+    };
+    int64_t jb = 0;
+    for (; jb + U * 256 <= n; jb += U * 256) {
+      if (jb < fhi && jb + U * 256 > flo) block(jb, std::true_type{});
+      else block(jb, std::false_type{});
     }
     for (; jb < n; jb += 256) {
       const int64_t j = jb + (int64_t)lane * 4;
@@ -561,13 +577,13 @@ __global__ __launch_bounds__(256) void k_update_seeds(float* __restrict__ D,
         if (mrow) mv = bf16x4(mrow + j);
         if (DEF == 2) pj = *reinterpret_cast<const v4f*>(xp + j);
       }
-      upd(dv, lv, xj, mv, pj, j, in);
+      upd(dv, lv, xj, mv, pj, j, in, std::true_type{});
       if (in) {
         if (DEF != 1) __builtin_nontemporal_store(dv, reinterpret_cast<v4f*>(row + j));
         if (COMP) *reinterpret_cast<v4f*>(rlo + j) = lv;
       }
     }
-    sp.write_ranked(seeds + i * 128);
+    if (SEEDS) sp.write_ranked(seeds + i * 128);
     if (rl.idx && lane == 0) {
       rl.count[i] = Tq == -kInf ? -1 : (int)lcount;
       rl.bound[i] = Tq;
@@ -587,11 +603,14 @@ __global__ __launch_bounds__(256) void k_compact(const float* __restrict__ X,
   if (blockIdx.x == 0 && threadIdx.x == 0) st->pending = 0;
   if (st->stopped) return;
   const int nc = st->ncand;
+  // slots of the candidate blocks that hold live candidates (the searches
+  // return before reading a block past the last live candidate)
+  const int cw = (nc + 63) / 64 * 64 < cpad ? (nc + 63) / 64 * 64 : cpad;
   // rows across blocks, candidate slots across threads (no index division)
   for (int64_t j = blockIdx.x; j < n; j += gridDim.x) {
     const float* __restrict__ xr = X + j * d;
     float* __restrict__ out = Xc + j * cpad;
-    for (int c = threadIdx.x; c < cpad; c += blockDim.x) out[c] = c < nc ? xr[cand[c]] : 0.0f;
+    for (int c = threadIdx.x; c < cw; c += blockDim.x) out[c] = c < nc ? xr[cand[c]] : 0.0f;
   }
 }
 
@@ -1341,21 +1360,26 @@ hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
                                const RowList& rl, const float* tauw, const float* frange, int d,
                                const int* cand, const int* iperm, const unsigned short* M16,
                                hipStream_t s, int defer) {
-  if (n % 4 || defer < 0 || defer > 2 || (defer && Dlo)) return hipErrorInvalidValue;
+  // backward lists with warm bounds need no new seeds (k_update_seeds)
+  const bool no_seeds = mult < 0 && rl.idx && tauw && !Dlo;
+  if (n % 4 || defer < 0 || defer > 2 || (defer && !no_seeds)) return hipErrorInvalidValue;
   const int64_t rows = (n + 3) / 4;
   const int g = (int)(rows < 8192 ? rows : 8192);
   if (Dlo)
-    hipLaunchKernelGGL((k_update_seeds<true, 0>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st,
-                       mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
+    hipLaunchKernelGGL((k_update_seeds<true, 0, true>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n,
+                       st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
   else if (defer == 1)
-    hipLaunchKernelGGL((k_update_seeds<false, 1>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st,
-                       mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
+    hipLaunchKernelGGL((k_update_seeds<false, 1, false>), dim3(g), dim3(256), 0, s, D, Dlo, XT,
+                       n, st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
   else if (defer == 2)
-    hipLaunchKernelGGL((k_update_seeds<false, 2>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st,
-                       mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
+    hipLaunchKernelGGL((k_update_seeds<false, 2, false>), dim3(g), dim3(256), 0, s, D, Dlo, XT,
+                       n, st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
+  else if (no_seeds)
+    hipLaunchKernelGGL((k_update_seeds<false, 0, false>), dim3(g), dim3(256), 0, s, D, Dlo, XT,
+                       n, st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
   else
-    hipLaunchKernelGGL((k_update_seeds<false, 0>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st,
-                       mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
+    hipLaunchKernelGGL((k_update_seeds<false, 0, true>), dim3(g), dim3(256), 0, s, D, Dlo, XT,
+                       n, st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
   return hipGetLastError();
 }
 
