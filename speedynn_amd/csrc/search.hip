@@ -27,18 +27,20 @@ namespace snn {
 // tie rule (speedynn/lib/cpu.c:149-165: strict '>' entry, strict '<' bubble):
 // an equal distance never displaces an earlier, lower index.  Branch-free
 // network over compile-time slots, so the list lives in VGPRs.
+// Callers pass only dd < Ld[K - 1] (so dd is not NaN).  Slot m's new distance
+// is dd clamped to [Ld[m-1], Ld[m]] (one v_med3: the list is sorted), which is
+// the value the compare-and-shift gives; its index follows the comparisons.
 template <int K>
 __device__ __forceinline__ void topk_insert(float (&Ld)[K], int (&Li)[K], float dd, int j) {
+  bool c[K];  // dd < Ld[m] on the list as it stands
+#pragma unroll
+  for (int m = 0; m < K; ++m) c[m] = dd < Ld[m];
 #pragma unroll
   for (int m = K - 1; m > 0; --m) {
-    const bool cp = dd < Ld[m - 1];
-    const bool ch = dd < Ld[m];
-    const float nd = cp ? Ld[m - 1] : (ch ? dd : Ld[m]);
-    const int ni = cp ? Li[m - 1] : (ch ? j : Li[m]);
-    Ld[m] = nd;
-    Li[m] = ni;
+    Li[m] = c[m - 1] ? Li[m - 1] : (c[m] ? j : Li[m]);
+    Ld[m] = __builtin_amdgcn_fmed3f(Ld[m - 1], dd, Ld[m]);
   }
-  if (dd < Ld[0]) {
+  if (c[0]) {
     Ld[0] = dd;
     Li[0] = j;
   }
