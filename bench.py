@@ -159,6 +159,11 @@ def pruned_kernel_names(pruned_kernel: int, K: int, mult: int, nblk: int) -> str
         names.append(f"k_search_pruned_shared<{K},{mult},{4 if nblk <= 8 else 8}> x2")
     if pruned_kernel & 8:  # the second step's walk along a sorted column
         names.append(f"k_search_walk<{K}>")
+    for bit, width in ((16, 4), (32, 8), (64, 16)):  # the tuner's workgroup widths
+        if pruned_kernel & bit:
+            parts = (nblk + width - 1) // width
+            names.append(f"k_search_pruned_shared<{K},{mult},{width}>"
+                         + (f" x{parts}" if parts > 1 else ""))
     return " + ".join(names) or f"k_search_pruned<{K},{mult}>"
 
 

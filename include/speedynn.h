@@ -110,7 +110,9 @@ typedef struct {
     int pruned_kernel;   /* pruned searches used, bit mask: 1 a wave per (query,
                             candidate block), 2 a workgroup per query sharing the row
                             scan, 4 that scan split over two workgroups per query,
-                            8 the second forward step's walk along a sorted column */
+                            8 the second forward step's walk along a sorted column,
+                            16 / 32 / 64 the shared scan in workgroups of 4 / 8 / 16
+                            waves (the autotuner's tile widths) */
     double ms_total;     /* device time of the selection (HIP events) */
     double ms_distance;  /* (a) initial matrix */
     double ms_search;    /* (c) candidate search kernels only */

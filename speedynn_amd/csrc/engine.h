@@ -74,7 +74,7 @@ struct EventPool {
   static constexpr int kRingSteps = 64;
   static constexpr int kMarks = 5;
   std::vector<hipEvent_t> ring;  // kRingSteps * kMarks
-  hipEvent_t tune[5] = {};
+  hipEvent_t tune[7] = {};
   hipEvent_t span[3] = {};
   bool ready = false;
 };

@@ -152,11 +152,12 @@ void destroy_events(snn_ctx* ctx) {
   ep = EventPool{};
 }
 
-// Autotuner of the pruned search.  Which kernel is faster -- a wave per
-// (query, candidate block), a workgroup per query sharing the row scan, or
-// that scan split over two workgroups per query -- depends on the data (how
-// much the candidate blocks' bounds differ) and on the step (early steps keep
-// many points, late ones few), not only on the shape.  The first step of a
+// Autotuner of the pruned search over its tile shapes.  Which is faster -- a
+// wave per (query, candidate block), or a workgroup of W = 4, 8 or 16 waves
+// per (query, W candidate blocks) sharing the row scan (ceil(blocks / W)
+// workgroups per query) -- depends on the data (how much the candidate
+// blocks' bounds differ: a workgroup waits for its loosest block) and on the
+// step (early steps keep many points, late ones few), not only on the shape.  The first step of a
 // shape (n, d, list size, direction) and step range [2^b, 2^(b+1)) runs every
 // applicable kernel on the same step, timed with HIP events, and the fastest
 // serves the rest of that range.  With nc kernels to try, kernel r serves the
@@ -201,7 +202,7 @@ void load_tuned_locked() {
     if (std::sscanf(line, " { \"n\": %lld, \"d\": %d, \"list\": %d, \"mult\": %d, \"range\": %d, "
                           "\"kernel\": %d",
                     &n, &d, &K, &mult, &b, &v) == 6 &&
-        v >= 1 && v <= 4)
+        v >= 1 && v <= 7)
       g_tuned[std::make_tuple((int64_t)n, d, K, mult, b)] = v;
   }
   std::fclose(f);
@@ -240,7 +241,7 @@ hipError_t launch_variant(const snn::SearchProblem& sl, int variant, const int* 
 struct PrunedPlan {
   int variant = 0;
   bool tune = false;
-  int cands[4] = {1, 0, 0, 0};
+  int cands[6] = {1, 0, 0, 0, 0, 0};
   int nc = 1;
   int bucket = 0;
   TuneKey key;
@@ -256,19 +257,29 @@ PrunedPlan plan_pruned(const snn::SearchProblem& sl, int d, int step, const int*
   while ((2 << pp.bucket) <= step + 1) ++pp.bucket;
   if (rank) pp.bucket = 64;
   pp.key = std::make_tuple(sl.n, d, snn::search_list_size(sl.kmax), sl.mult, pp.bucket);
-  // the kernels that apply to this shape
-  if (snn::pruned_shared_applies(sl)) pp.cands[pp.nc++] = 2;
-  if (snn::pruned_split_applies(sl)) pp.cands[pp.nc++] = 3;
+  // the tile shapes that apply: a wave per (query, candidate block), and the
+  // shared row scan in workgroups of W = 4, 8, 16 waves (one query x W
+  // candidate blocks; a width is tried where more than half of its waves
+  // hold candidate blocks), then the second forward step's walk
+  const int nblk = snn::live_blocks(sl);
+  for (int W = 4; W <= 16; W *= 2)
+    if (snn::pruned_width_applies(sl, W) && (W == 4 || 2 * nblk > W))
+      pp.cands[pp.nc++] = W == 4 ? 5 : W == 8 ? 6 : 7;
   if (rank) pp.cands[pp.nc++] = 4;
-  // SPEEDYNN_PRUNED=wave|shared|split|walk runs that kernel wherever it
-  // applies (read per call, for measurements and the per-kernel tests)
+  // SPEEDYNN_PRUNED=wave|shared|split|walk|shared4|shared8|shared16 runs that
+  // kernel wherever it applies (read per call, for measurements and the
+  // per-kernel tests); shared and split take the width by rule
   const char* fe = std::getenv("SPEEDYNN_PRUNED");
   if (fe && *fe) {
-    int f = !std::strcmp(fe, "wave") ? 1 : !std::strcmp(fe, "shared") ? 2
-          : !std::strcmp(fe, "split") ? 3 : !std::strcmp(fe, "walk") ? 4 : 0;
-    for (int i = 0; i < pp.nc; ++i)
-      if (pp.cands[i] == f) pp.variant = f;
-    if (pp.variant == 0 && f != 4) pp.variant = 1;  // the per-wave kernel applies everywhere
+    const int f = !std::strcmp(fe, "wave") ? 1 : !std::strcmp(fe, "shared") ? 2
+                : !std::strcmp(fe, "split") ? 3 : !std::strcmp(fe, "walk") ? 4
+                : !std::strcmp(fe, "shared4") ? 5 : !std::strcmp(fe, "shared8") ? 6
+                : !std::strcmp(fe, "shared16") ? 7 : 0;
+    if (f == 2 && snn::pruned_shared_applies(sl)) pp.variant = 2;
+    else if (f == 3 && snn::pruned_split_applies(sl)) pp.variant = 3;
+    else if (f == 4 && rank) pp.variant = 4;
+    else if (f >= 5 && snn::pruned_width_applies(sl, 4 << (f - 5))) pp.variant = f;
+    else if (f != 4) pp.variant = 1;  // the per-wave kernel applies everywhere
     return pp;
   }
   if (!off && pp.nc > 1) {
@@ -291,7 +302,9 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int
   int variant = pp.variant;
   if (!pp.tune) {
     HIPCHK(launch_variant(sl, variant, rank, s));
-    const int v = variant ? variant : (sl.mult > 0 || !snn::pruned_shared_applies(sl) ? 1 : 2);
+    int v = variant ? variant : (sl.mult > 0 || !snn::pruned_shared_applies(sl) ? 1 : 2);
+    if (v >= 5 && !snn::pruned_width_applies(sl, 4 << (v - 5))) v = 1;
+    if (v == 3 && !snn::pruned_split_applies(sl)) v = 1;
     *used |= 1 << (v - 1);
     return SNN_OK;
   }
@@ -300,7 +313,7 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int
   const TuneKey key = pp.key;
   // the kernels' order rotates with the range
   {
-    int rot[4];
+    int rot[6];
     for (int i = 0; i < nc; ++i) rot[i] = cands[(i + bucket) % nc];
     for (int i = 0; i < nc; ++i) cands[i] = rot[i];
   }
@@ -323,7 +336,7 @@ int search_pruned_tuned(const snn::SearchProblem& sl, int d, int step, const int
   }
   HIPCHK(hipEventSynchronize(ev[nc]));
   variant = cands[0];
-  double t[4];
+  double t[6];
   double best = t[0] = elapsed(ev[0], ev[1]);
   for (int i = 1; i < nc; ++i) {
     t[i] = elapsed(ev[i], ev[i + 1]);

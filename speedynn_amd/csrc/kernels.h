@@ -43,9 +43,12 @@ hipError_t launch_search_long(const SearchProblem& a, float* ld, int* li, float*
 // variant: 0 = by direction (forward: a wave per (query, candidate block);
 // backward: a workgroup per query sharing the row scan), 1 = per wave,
 // 2 = shared scan (where pruned_shared_applies), 3 = shared scan split over
-// two workgroups per query (where pruned_split_applies).
+// two workgroups per query (where pruned_split_applies), 5 / 6 / 7 = shared
+// scan in workgroups of 4 / 8 / 16 waves, ceil(blocks / W) per query (where
+// pruned_width_applies; else per wave).
 hipError_t launch_search_pruned(const SearchProblem& a, int variant, hipStream_t s);
 bool pruned_shared_applies(const SearchProblem& a);
+bool pruned_width_applies(const SearchProblem& a, int W);
 bool pruned_split_applies(const SearchProblem& a);
 // Second forward step of the precision form: the walk along the first chosen
 // feature's sorted column (a.skey, a.sidx; rank from launch_rank after the

@@ -1580,6 +1580,20 @@ static hipError_t launch_pruned_k(const SearchProblem& a, int variant, hipStream
   // for its loosest block), backward the shared scan (two rows per query,
   // similar bounds)
   const int v = variant;
+  // variants 5, 6, 7: the shared scan with workgroups of W = 4, 8, 16 waves,
+  // ceil(nblk / W) workgroups per query (the tile of one workgroup: one query
+  // x W candidate blocks)
+  if (v >= 5 && v <= 7) {
+    const int Wv = 4 << (v - 5);
+    if constexpr (K <= 50) {
+      if (pruned_width_applies(a, Wv)) {
+        if (Wv == 4) launch_pruned_shared<K, 4>(a, s);
+        else if (Wv == 8) launch_pruned_shared<K, 8>(a, s);
+        else if constexpr (K <= 20) launch_pruned_shared<K, 16>(a, s);
+        return hipGetLastError();
+      }
+    }
+  }
   const bool shared = K <= 50 && W > 0 && (v == 2 || (v == 0 && a.mult < 0));
   if (shared) {
     if constexpr (K <= 50) {
@@ -1681,6 +1695,11 @@ hipError_t launch_rank(const int* sidx, const SelState* st, int* rank, int64_t n
   hipLaunchKernelGGL(k_rank, dim3((unsigned)(g < 4096 ? g : 4096)), dim3(256), 0, s, sidx, st,
                      rank, n);
   return hipGetLastError();
+}
+
+bool pruned_width_applies(const SearchProblem& a, int W) {
+  const int K = search_list_size(a.kmax);
+  return K > 0 && K <= 50 && (W == 4 || W == 8 || (W == 16 && K <= 20));
 }
 
 bool pruned_split_applies(const SearchProblem& a) {

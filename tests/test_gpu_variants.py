@@ -1,5 +1,6 @@
 """Every pruned-search kernel against the CPU oracle: SPEEDYNN_PRUNED forces
-one kernel (per-wave, shared scan, split shared scan, sorted-column walk) for
+one kernel (per-wave, shared scan, split shared scan, sorted-column walk, the
+shared scan at each workgroup width the autotuner tries) for
 every step where it applies, so each is checked on its own rather than only
 when the autotuner picks it.  Shapes are chosen so that each kernel applies:
 more than four candidate blocks for the split scan, a second forward step with
@@ -42,7 +43,8 @@ def _data(seed, n, d, model):
     return X, y
 
 
-@pytest.mark.parametrize("kernel", ["wave", "shared", "split", "walk"])
+@pytest.mark.parametrize("kernel", ["wave", "shared", "split", "walk", "shared4", "shared8",
+                                    "shared16"])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"seed{c[0]}")
 def test_forced_pruned_kernel_matches_oracle(engine, monkeypatch, case, kernel):
     seed, n, d, k, method, model, nf = case
@@ -59,12 +61,16 @@ def test_forced_pruned_kernel_matches_oracle(engine, monkeypatch, case, kernel):
     np.testing.assert_array_equal(g["candidate_errors"][:steps].view(np.uint32),
                                   o["candidate_errors"][:steps].view(np.uint32))
     used = g["pruned_kernel"]
-    want = {"wave": 1, "shared": 2, "split": 4, "walk": 8}[kernel]
+    want = {"wave": 1, "shared": 2, "split": 4, "walk": 8, "shared4": 16, "shared8": 32,
+            "shared16": 64}[kernel]
     applies = {
         "wave": True,
         "shared": True,
         "split": d > 256,
         "walk": method == "forward",
+        "shared4": True,
+        "shared8": True,
+        "shared16": k <= 20,
     }[kernel]
     if applies:
         assert used & want, (kernel, used)
@@ -75,7 +81,8 @@ def test_autotuned_selection_matches_oracle(engine, monkeypatch, case):
     """The autotuner on (no kernel forced): the first step of each step range
     runs every applicable kernel, kernel r on the queries r, r + nc, ... of
     that step, so that step's results come from several kernels at once
-    (up to four here: per-wave, shared, split and the walk).  Every step,
+    (up to five here: per-wave, the shared scan at widths 4, 8 and 16, and the
+    walk).  Every step,
     tuning or not, must give the oracle's errors bit for bit.  These shapes
     appear in no other autotuned test, so the tuning steps run here."""
     seed, n, d, k, method, model, nf = case
@@ -99,7 +106,8 @@ REDUCED = [os.path.join(GOLDEN, f) for f in ("c2_n3000_d512_oracle.json",
                                              "c4_n1000_d320_oracle.json")]
 
 
-@pytest.mark.parametrize("kernel", ["wave", "shared", "split", "walk"])
+@pytest.mark.parametrize("kernel", ["wave", "shared", "split", "walk", "shared4", "shared8",
+                                    "shared16"])
 @pytest.mark.parametrize("path", REDUCED, ids=lambda p: os.path.basename(p)[:13])
 def test_forced_kernel_on_workload_recipes_matches_oracle(engine, monkeypatch, path, kernel):
     """Each pruned kernel forced on the n=65536 forward recipe (feature scales
