@@ -87,15 +87,59 @@ __device__ __forceinline__ float comp_error(float a, float b, float t, float p, 
 // features staged in LDS.
 // With MAXM, M[i][j] = max over features of fl((x_ik - x_jk)^2) is written
 // too: the backward pruned search bounds d = fl(D - q_f) >= fl(D - M).
-template <bool COMP, bool MAXM>
+// Tile (bi, bj), bi <= bj, of the upper triangle of 64 x 64 tiles, from a
+// linear block index L (column-major: L = bj (bj + 1) / 2 + bi).
+__device__ __forceinline__ void tri_tile(int64_t L, int64_t& bi, int64_t& bj) {
+  int64_t c = (int64_t)((sqrt(8.0 * (double)L + 1.0) - 1.0) / 2.0);
+  while (c * (c + 1) / 2 > L) --c;
+  while ((c + 1) * (c + 2) / 2 <= L) ++c;
+  bj = c;
+  bi = L - c * (c + 1) / 2;
+}
+
+// A 64 x 64 tile's values v[a][b] at (i0 + 4ty + a, j0 + 4tx + b) stored at
+// their place and, for an off-diagonal tile of a symmetric matrix, mirrored
+// at (j, i): each thread's mirrored values of one column j are four
+// consecutive floats of row j.
+__device__ __forceinline__ void store_tile(float* __restrict__ Out, int64_t n, int64_t i0,
+                                           int64_t j0, int tx, int ty, const float (&v)[4][4],
+                                           bool mirror) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int64_t i = i0 + ty * 4 + a;
+    if (i >= n) continue;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int64_t j = j0 + tx * 4 + b;
+      if (j < n) Out[i * n + j] = v[a][b];
+    }
+  }
+  if (!mirror) return;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int64_t j = j0 + tx * 4 + b;
+    if (j >= n) continue;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int64_t i = i0 + ty * 4 + a;
+      if (i < n) Out[j * n + i] = v[a][b];
+    }
+  }
+}
+
+template <bool COMP, bool MAXM, bool TRI>
 __global__ __launch_bounds__(256) void k_gram_exact(const float* __restrict__ XT,
                                                     float* __restrict__ D, float* __restrict__ Dlo,
                                                     float* __restrict__ M, int64_t n, int d) {
   constexpr int KC = 16;
   __shared__ float As[KC][64];
   __shared__ float Bs[KC][64];
-  const int64_t i0 = (int64_t)blockIdx.y * 64;
-  const int64_t j0 = (int64_t)blockIdx.x * 64;
+  // TRI: one tile of each symmetric pair (D and M are exactly symmetric:
+  // fl(a - b) = -fl(b - a), so every term's square is the same), mirrored
+  int64_t bi = blockIdx.y, bj = blockIdx.x;
+  if (TRI) tri_tile(blockIdx.x, bi, bj);
+  const int64_t i0 = bi * 64;
+  const int64_t j0 = bj * 64;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   float acc[4][4], lo[4][4], mx[4][4];
 #pragma unroll
@@ -138,20 +182,9 @@ __global__ __launch_bounds__(256) void k_gram_exact(const float* __restrict__ XT
     }
     __syncthreads();
   }
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int64_t i = i0 + ty * 4 + a;
-    if (i >= n) continue;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int64_t j = j0 + tx * 4 + b;
-      if (j < n) {
-        D[i * n + j] = acc[a][b];
-        if (COMP) Dlo[i * n + j] = lo[a][b];
-        if (MAXM) M[i * n + j] = mx[a][b];
-      }
-    }
-  }
+  store_tile(D, n, i0, j0, tx, ty, acc, TRI && bi != bj);
+  if (COMP) store_tile(Dlo, n, i0, j0, tx, ty, lo, false);
+  if (MAXM) store_tile(M, n, i0, j0, tx, ty, mx, TRI && bi != bj);
 }
 
 // M16[i] = M[i] rounded up to bfloat16 (M >= 0; the carry of the rounding
@@ -193,8 +226,11 @@ __global__ __launch_bounds__(256) void k_max_square(const float* __restrict__ Xc
   __shared__ float Bs[KC][64];
   if (st->stopped) return;
   const int ncand = st->ncand;
-  const int64_t i0 = (int64_t)blockIdx.y * 64;
-  const int64_t j0 = (int64_t)blockIdx.x * 64;
+  // one tile of each symmetric pair, mirrored (M is symmetric)
+  int64_t bi, bj;
+  tri_tile(blockIdx.x, bi, bj);
+  const int64_t i0 = bi * 64;
+  const int64_t j0 = bj * 64;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   float mx[4][4];
 #pragma unroll
@@ -231,16 +267,7 @@ __global__ __launch_bounds__(256) void k_max_square(const float* __restrict__ Xc
     }
     __syncthreads();
   }
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int64_t i = i0 + ty * 4 + a;
-    if (i >= n) continue;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int64_t j = j0 + tx * 4 + b;
-      if (j < n) M[i * n + j] = mx[a][b];
-    }
-  }
+  store_tile(M, n, i0, j0, tx, ty, mx, bi != bj);
 }
 
 // --------------------------------------------------------- (b) update
@@ -704,6 +731,9 @@ __device__ __forceinline__ float lane_seq_sum(const float* __restrict__ P0,
   const int wave = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int64_t ntile = (n + kLossTile - 1) / kLossTile;
   float s = 0.0f;
+  // the adder's chain of dependent adds is the critical path: its wave wins
+  // issue over the loaders that share its SIMD
+  if (wave == kLossLoaders) __builtin_amdgcn_s_setprio(3);
   // loaders keep tile u in register slot u % kLossAhead: the loads of a tile
   // are issued kLossAhead - 1 tiles before it is staged, which hides the
   // memory latency behind the adder's work on the tiles in between
@@ -799,6 +829,7 @@ __device__ __forceinline__ float lane_seq_sum(const float* __restrict__ P0,
     tile_step(t0 + 2, std::integral_constant<int, 2>{}, No{});
     tile_step(t0 + 3, std::integral_constant<int, 3>{}, No{});
   }
+  if (wave == kLossLoaders) __builtin_amdgcn_s_setprio(0);
   return s;
 }
 
@@ -1317,12 +1348,15 @@ hipError_t launch_transpose(const float* X, float* XT, int64_t n, int d, hipStre
 
 hipError_t launch_gram_exact(const float* XT, float* D, float* Dlo, float* M, int64_t n, int d,
                              hipStream_t s) {
-  dim3 g((unsigned)((n + 63) / 64), (unsigned)((n + 63) / 64));
+  // the compensated Gram keeps the whole grid; otherwise the upper triangle
+  // of T x T tiles, each mirrored
+  const int64_t T = (n + 63) / 64;
+  dim3 g((unsigned)T, (unsigned)T), gt((unsigned)(T * (T + 1) / 2));
   dim3 b(256);
-  if (Dlo && M) hipLaunchKernelGGL((k_gram_exact<true, true>), g, b, 0, s, XT, D, Dlo, M, n, d);
-  else if (Dlo) hipLaunchKernelGGL((k_gram_exact<true, false>), g, b, 0, s, XT, D, Dlo, M, n, d);
-  else if (M) hipLaunchKernelGGL((k_gram_exact<false, true>), g, b, 0, s, XT, D, Dlo, M, n, d);
-  else hipLaunchKernelGGL((k_gram_exact<false, false>), g, b, 0, s, XT, D, Dlo, M, n, d);
+  if (Dlo && M) hipLaunchKernelGGL((k_gram_exact<true, true, false>), g, b, 0, s, XT, D, Dlo, M, n, d);
+  else if (Dlo) hipLaunchKernelGGL((k_gram_exact<true, false, false>), g, b, 0, s, XT, D, Dlo, M, n, d);
+  else if (M) hipLaunchKernelGGL((k_gram_exact<false, true, true>), gt, b, 0, s, XT, D, Dlo, M, n, d);
+  else hipLaunchKernelGGL((k_gram_exact<false, false, true>), gt, b, 0, s, XT, D, Dlo, M, n, d);
   return hipGetLastError();
 }
 
@@ -1336,7 +1370,8 @@ hipError_t launch_round_up_bf16(const float* M, unsigned short* M16, int64_t cou
 
 hipError_t launch_max_square(const float* Xc, const SelState* st, float* M, int64_t n, int cpad,
                              hipStream_t s) {
-  dim3 g((unsigned)((n + 63) / 64), (unsigned)((n + 63) / 64));
+  const int64_t T = (n + 63) / 64;  // upper triangle of T x T tiles
+  dim3 g((unsigned)(T * (T + 1) / 2));
   hipLaunchKernelGGL(k_max_square, g, dim3(256), 0, s, Xc, st, M, n, cpad);
   return hipGetLastError();
 }
