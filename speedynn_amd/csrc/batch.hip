@@ -159,7 +159,8 @@ struct Plan {
       if (sl.fused)
         HIPCHK(snn::launch_update_seeds(at<float>(A(), sl.D), Dlo, at<float>(A(), sl.XT), pb.n,
                                         st, mult, at<int>(A(), sl.seeds), pb.npf, snn::RowList{},
-                                        nullptr, nullptr, pb.d, nullptr, nullptr, nullptr, s));
+                                        nullptr, nullptr, pb.d, nullptr, nullptr, nullptr,
+                                        nullptr, s));
       else
         HIPCHK(snn::launch_update(at<float>(A(), sl.D), Dlo, at<float>(A(), sl.XT), pb.n, st, -1,
                                   mult, s));

@@ -100,7 +100,7 @@ struct snn_ctx {
   hipStream_t stream = nullptr;
   snn::DevBuf X, y, XT, D, Dlo, Xc, P, err, val, bestk, cand, sel, ordering, errors, cand_err,
       kvals, losses, state, scratch, seeds, visited, perm, key, M, med, skey, sidx, sort_tmp,
-      rank, tauw, iperm, frange, lidx, lval, llb, lcount, M16, long_d, long_i, long_lab, lbound;
+      rank, tauw, iperm, frange, lidx, lval, llb, lcount, Mq, long_d, long_i, long_lab, lbound;
   snn::BatchCache batch;
   EventPool events;
   StepGraphs step_graphs;

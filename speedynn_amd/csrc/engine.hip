@@ -365,7 +365,8 @@ struct Work {
   int *lidx = nullptr, *lcount = nullptr;
   float* lbound = nullptr;
   float *lval = nullptr, *llb = nullptr;
-  unsigned short* M16 = nullptr;
+  unsigned char* Mq = nullptr;  // M rounded up to 8 bits (n * n), then the row scales (n ints)
+  int* Mexp = nullptr;
   int *seeds = nullptr, *sidx = nullptr, *rank = nullptr;
   float *long_d = nullptr, *long_lab = nullptr;
   int* long_i = nullptr;
@@ -524,7 +525,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     bd.add(ctx->lval, sizeof(float) * (size_t)n * rows_cap, &w.lval);
     if (!fwd) {
       bd.add(ctx->llb, sizeof(float) * (size_t)n * rows_cap, &w.llb);
-      bd.add(ctx->M16, sizeof(unsigned short) * nn, &w.M16);
+      bd.add(ctx->Mq, (size_t)nn + sizeof(int) * (size_t)n, &w.Mq);
     }
     bd.add(ctx->lcount, sizeof(int) * (size_t)n, &w.lcount);
     bd.add(ctx->lbound, sizeof(float) * (size_t)n, &w.lbound);
@@ -547,6 +548,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   }
   rc = bd.bind();
   if (rc) return rc;
+  if (w.Mq) w.Mexp = reinterpret_cast<int*>(w.Mq + nn);  // nn % 16 == 0 (n % 4 == 0)
   if (timed) {
     rc = ensure_events(ctx);
     if (rc) return rc;
@@ -583,7 +585,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     if (Dlo) HIPCHK(snn::launch_fill(Dlo, (int64_t)nn, 0.0f, s));
   } else {
     HIPCHK(snn::launch_gram_exact(w.XT, D, Dlo, pruned ? w.M : nullptr, n, d, s));
-    if (w.M16) HIPCHK(snn::launch_round_up_bf16(w.M, w.M16, (int64_t)nn, s));
+    if (w.Mq) HIPCHK(snn::launch_quantize_m(w.M, w.Mq, w.Mexp, n, s));
   }
   if (timed) HIPCHK(hipEventRecord(e1, s));
 
@@ -671,7 +673,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     const bool pruned_step = pruned && (step > 0 || !fwd);
     if (refresh) {
       HIPCHK(snn::launch_max_square(w.Xc, st, w.M, n, pb.cpad, s));
-      if (w.M16) HIPCHK(snn::launch_round_up_bf16(w.M, w.M16, (int64_t)nn, s));
+      if (w.Mq) HIPCHK(snn::launch_quantize_m(w.M, w.Mq, w.Mexp, n, s));
     }
     // seeds: the previous step's update wrote them where it was fused
     if (pruned_step && !(fused_seeds && step > 0))
@@ -710,7 +712,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     if (step + 1 < pb.max_steps) {
       if (fused_seeds)
         HIPCHK(snn::launch_update_seeds(D, Dlo, w.XT, n, st, fwd ? 1 : -1, w.seeds, pb.npf,
-                                        sl.rows, w.tauw, w.frange, d, w.cand, w.iperm, w.M16,
+                                        sl.rows, w.tauw, w.frange, d, w.cand, w.iperm, w.Mq, w.Mexp,
                                         s, update_mode(step)));
       else
         HIPCHK(snn::launch_update(D, Dlo, w.XT, n, st, -1, fwd ? 1 : -1, s));
@@ -724,7 +726,7 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     const void* ptrs[] = {dX, dy, w.XT, w.D, w.Dlo, w.Xc, w.P, w.err, w.val, w.errors,
                           w.cand_err, w.bestk, w.cand, w.sel, w.ordering, w.kvals, w.losses,
                           w.perm, w.iperm, w.st, w.key, w.M, w.skey, w.med, w.tauw, w.frange,
-                          w.lidx, w.lcount, w.lbound, w.lval, w.llb, w.M16, w.seeds, w.sidx, w.rank,
+                          w.lidx, w.lcount, w.lbound, w.lval, w.llb, w.Mq, w.seeds, w.sidx, w.rank,
                           w.visited, w.long_d, w.long_i, w.long_lab};
     for (const void* q : ptrs) gbase.push_back((size_t)q);
     const int64_t vals[] = {n, d, pb.npf, pb.cpad, num_k, num_loss, kmax, pb.method,
@@ -950,7 +952,7 @@ void snn_destroy(snn_ctx* c) {
                     &c->kvals, &c->losses, &c->state, &c->scratch, &c->seeds,
                     &c->visited, &c->perm, &c->key, &c->M, &c->med, &c->skey,
                     &c->sidx, &c->sort_tmp, &c->rank, &c->tauw, &c->iperm, &c->frange,
-                    &c->lidx, &c->lval, &c->llb, &c->lcount, &c->M16, &c->long_d,
+                    &c->lidx, &c->lval, &c->llb, &c->lcount, &c->Mq, &c->long_d,
                     &c->long_i, &c->long_lab, &c->lbound};
   for (DevBuf* b : bufs) b->release();
   destroy_events(c);

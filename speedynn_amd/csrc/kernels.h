@@ -18,17 +18,18 @@ hipError_t launch_max_square(const float* Xc, const SelState* st, float* M, int6
 // of the next step (seeds as launch_seed_scan writes them); n % 4 == 0.
 // rl.idx set also writes the next search's row lists (RowList) from the warm
 // bounds tauw (forward: and the chosen column's range frange; backward: the
-// lower bounds fl(D - M16) from M16 = M rounded up to bfloat16).
+// lower bounds fl(D - Mq) from Mq = M rounded up to 8 bits, launch_quantize_m).
 // defer: 0 write D; 1 leave this step's term out of D (st->fdefer); 2 apply
 // the left-out term and this step's, and write D (see k_update_seeds).
 hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
                                SelState* st, int mult, int* seeds, int64_t npf,
                                const RowList& rl, const float* tauw, const float* frange, int d,
-                               const int* cand, const int* iperm, const unsigned short* M16,
-                               hipStream_t s, int defer = 0);
-// M16 = M rounded up to bfloat16, count values.
-hipError_t launch_round_up_bf16(const float* M, unsigned short* M16, int64_t count,
-                                hipStream_t s);
+                               const int* cand, const int* iperm, const unsigned char* Mq,
+                               const int* Mexp, hipStream_t s, int defer = 0);
+// Mq[i][j] = M[i][j] rounded up to 8 bits with the row scale Mexp[i] (n % 4 == 0;
+// see k_quantize_m for the code).
+hipError_t launch_quantize_m(const float* M, unsigned char* Mq, int* Mexp, int64_t n,
+                             hipStream_t s);
 hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const SelState* st,
                          int f, int mult, hipStream_t s);
 hipError_t launch_compact(const float* X, const int* cand, SelState* st, float* Xc, int64_t n,

@@ -187,31 +187,6 @@ __global__ __launch_bounds__(256) void k_gram_exact(const float* __restrict__ XT
   if (MAXM) store_tile(M, n, i0, j0, tx, ty, mx, TRI && bi != bj);
 }
 
-// M16[i] = M[i] rounded up to bfloat16 (M >= 0; the carry of the rounding
-// moves into the exponent), the bound the distance update reads to list a
-// backward row's points: M16 >= M, so fl(D - M16) <= fl(D - M) is still a
-// lower bound of every remaining candidate's distance, at half the bytes.
-__global__ __launch_bounds__(256) void k_round_up_bf16(const float* __restrict__ M,
-                                                       unsigned short* __restrict__ M16,
-                                                       int64_t count) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x * 4;
-  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; i < count; i += stride) {
-    unsigned short h[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      uint32_t b = i + e < count ? __float_as_uint(M[i + e]) : 0u;
-      if (b & 0xFFFFu) b = (b + 0x10000u) & 0xFFFF0000u;
-      h[e] = (unsigned short)(b >> 16);
-    }
-    if (i + 3 < count) {
-      *reinterpret_cast<uint2*>(M16 + i) =
-          make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
-    } else {
-      for (int e = 0; e < 4 && i + e < count; ++e) M16[i + e] = h[e];
-    }
-  }
-}
-
 // M refreshed over the remaining candidates of a backward selection:
 // M[i][j] = max over c < ncand of fl((Xc[i][c] - Xc[j][c])^2), the same
 // squares the search subtracts, so fl(D - M) stays a lower bound of every
@@ -349,12 +324,87 @@ __global__ __launch_bounds__(256) void k_update(float* __restrict__ D, float* __
   }
 }
 
-// Four bfloat16 values (8-byte aligned) widened to float, streamed once.
-__device__ __forceinline__ v4f bf16x4(const unsigned short* p) {
-  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
-  const u2 w = __builtin_nontemporal_load(reinterpret_cast<const u2*>(p));
-  return v4f{__uint_as_float(w.x << 16), __uint_as_float(w.x & 0xFFFF0000u),
-             __uint_as_float(w.y << 16), __uint_as_float(w.y & 0xFFFF0000u)};
+// M rounded up to 8 bits per value with a scale per row, the bound the
+// distance update reads to list a backward row's points: code c of row i
+// stands for 2^(E_i - 16 + (c >> 4)) * (1 + (c & 15) / 16), the smallest such
+// value >= M_ij, with E_i = floor(log2 max_j M_ij) + 2 (at least -100), so a
+// row's values down to 2^-14 of its maximum keep four mantissa bits and the
+// smaller ones round up to 2^(E_i - 16).  Mq >= M, so fl(D - Mq) <= fl(D - M)
+// is still a lower bound of every remaining candidate's distance, at a
+// quarter of M's bytes.  A row whose maximum is not finite or reaches 2^126
+// has E_i = kMqInf and reads as +inf (every point listed: no list).
+constexpr int kMqInf = 1 << 20;
+
+__device__ __forceinline__ unsigned int mq_encode(float v, int E) {
+  const unsigned int b = __float_as_uint(v);
+  const int base = E - 16;
+  const int e = (int)((b >> 23) & 0xFFu) - 127;
+  if (e < base) return 0u;  // 2^base >= v
+  int ef = e - base;
+  unsigned int m = (b >> 19) & 15u;
+  if (b & 0x7FFFFu) {
+    if (++m == 16u) {
+      m = 0u;
+      ++ef;
+    }
+  }
+  return ef > 15 ? 255u : ((unsigned int)ef << 4) | m;  // ef > 15: only for non-finite M
+}
+
+__device__ __forceinline__ float mq_decode(unsigned int c, int E) {
+  return __uint_as_float(((unsigned int)(E - 16 + (int)(c >> 4) + 127) << 23) | ((c & 15u) << 19));
+}
+
+// Four codes (4-byte aligned): loaded once (streamed), widened to float.
+__device__ __forceinline__ unsigned int mq_raw(const unsigned char* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const unsigned int*>(p));
+}
+__device__ __forceinline__ v4f mq_widen(unsigned int w, int E) {
+  if (E == kMqInf) {
+    const float inf = __int_as_float(0x7f800000);
+    return v4f{inf, inf, inf, inf};
+  }
+  return v4f{mq_decode(w & 255u, E), mq_decode((w >> 8) & 255u, E),
+             mq_decode((w >> 16) & 255u, E), mq_decode(w >> 24, E)};
+}
+
+// One wave per row (n % 4 == 0): the row's maximum, then its codes.
+__global__ __launch_bounds__(256) void k_quantize_m(const float* __restrict__ M,
+                                                    unsigned char* __restrict__ Mq,
+                                                    int* __restrict__ Mexp, int64_t n) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t i = (int64_t)blockIdx.x * 4 + uniform(threadIdx.x >> 6); i < n;
+       i += (int64_t)gridDim.x * 4) {
+    const float* __restrict__ row = M + i * n;
+    float mx = 0.0f;
+    bool bad = false;
+    for (int64_t j = (int64_t)lane * 4; j < n; j += 256) {
+      const v4f v = *reinterpret_cast<const v4f*>(row + j);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        bad |= !(v[e] < 0x1p126f);  // NaN, inf or too large
+        mx = fmaxf(mx, v[e]);
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    int E;
+    if (__any(bad)) {
+      E = kMqInf;
+    } else {
+      const int em = (int)((__float_as_uint(mx) >> 23) & 0xFFu) - 127;
+      E = em + 2 > -100 ? em + 2 : -100;
+    }
+    if (lane == 0) Mexp[i] = E;
+    unsigned char* __restrict__ out = Mq + i * n;
+    for (int64_t j = (int64_t)lane * 4; j < n; j += 256) {
+      const v4f v = *reinterpret_cast<const v4f*>(row + j);
+      const unsigned int w = E == kMqInf ? 0u
+                             : mq_encode(v[0], E) | (mq_encode(v[1], E) << 8) |
+                                   (mq_encode(v[2], E) << 16) | (mq_encode(v[3], E) << 24);
+      *reinterpret_cast<unsigned int*>(out + j) = w;
+    }
+  }
 }
 
 // The update fused with the next step's seed scan (n % 4 == 0, feature from
@@ -369,7 +419,7 @@ __device__ __forceinline__ v4f bf16x4(const unsigned short* p) {
 // (cand[c < ncand], k_select has listed them): forward fl(max tau + Q)
 // widened as in the search's warm bound, backward max tau itself; no list
 // where some candidate's tau is unknown (kMaxF) or X is not finite.  The
-// backward lower bound fl(D - M16) uses M rounded up to bfloat16 (M16 >= M),
+// backward lower bound fl(D - Mq) uses M rounded up to 8 bits (Mq >= M),
 // from the rows as they stand; a later refresh only raises the bound, so the
 // list still holds every point the search can visit.
 //
@@ -394,7 +444,8 @@ void k_update_seeds(float* __restrict__ D,
                                                       const float* __restrict__ frange, int d,
                                                       const int* __restrict__ cand,
                                                       const int* __restrict__ iperm,
-                                                      const unsigned short* __restrict__ M16) {
+                                                      const unsigned char* __restrict__ Mq,
+                                                      const int* __restrict__ Mexp) {
   if (!st->pending) return;
   const int f = st->fstar;
   const int lane = threadIdx.x & 63;
@@ -500,7 +551,8 @@ void k_update_seeds(float* __restrict__ D,
     int* __restrict__ lidx = rl.idx ? rl.idx + i * rl.cap : nullptr;
     float* __restrict__ lval = rl.idx ? rl.val + i * rl.cap : nullptr;
     float* __restrict__ llb = rl.lb ? rl.lb + i * rl.cap : nullptr;
-    const unsigned short* __restrict__ mrow = rl.lb ? M16 + i * n : nullptr;  // backward lists
+    const unsigned char* __restrict__ mrow = rl.lb ? Mq + i * n : nullptr;  // backward lists
+    const int Erow = rl.lb ? Mexp[i] : 0;
     // in: the lane holds four points of the row (all lanes call this together);
     // mv: their M values (backward lists)
     v4f lbv = {0.f, 0.f, 0.f, 0.f};
@@ -575,7 +627,7 @@ void k_update_seeds(float* __restrict__ D,
         xj[u] = *reinterpret_cast<const v4f*>(xf + j + u * 256);
         dv[u] = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j + u * 256));
         if (COMP) lv[u] = *reinterpret_cast<v4f*>(rlo + j + u * 256);
-        mv[u] = mrow ? bf16x4(mrow + j + u * 256) : v4f{0.f, 0.f, 0.f, 0.f};
+        mv[u] = mrow ? mq_widen(mq_raw(mrow + j + u * 256), Erow) : v4f{0.f, 0.f, 0.f, 0.f};
         pj[u] = DEF == 2 ? *reinterpret_cast<const v4f*>(xp + j + u * 256)
                          : v4f{0.f, 0.f, 0.f, 0.f};
       }
@@ -601,7 +653,7 @@ void k_update_seeds(float* __restrict__ D,
         xj = *reinterpret_cast<const v4f*>(xf + j);
         dv = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j));
         if (COMP) lv = *reinterpret_cast<v4f*>(rlo + j);
-        if (mrow) mv = bf16x4(mrow + j);
+        if (mrow) mv = mq_widen(mq_raw(mrow + j), Erow);
         if (DEF == 2) pj = *reinterpret_cast<const v4f*>(xp + j);
       }
       upd(dv, lv, xj, mv, pj, j, in, std::true_type{});
@@ -1360,11 +1412,12 @@ hipError_t launch_gram_exact(const float* XT, float* D, float* Dlo, float* M, in
   return hipGetLastError();
 }
 
-hipError_t launch_round_up_bf16(const float* M, unsigned short* M16, int64_t count,
-                                hipStream_t s) {
-  const int64_t g = (count / 4 + 255) / 256;
-  hipLaunchKernelGGL(k_round_up_bf16, dim3((unsigned)(g < 16384 ? g : 16384)), dim3(256), 0, s,
-                     M, M16, count);
+hipError_t launch_quantize_m(const float* M, unsigned char* Mq, int* Mexp, int64_t n,
+                             hipStream_t s) {
+  if (n % 4) return hipErrorInvalidValue;
+  const int64_t rows = (n + 3) / 4;
+  hipLaunchKernelGGL(k_quantize_m, dim3((unsigned)(rows < 8192 ? rows : 8192)), dim3(256), 0, s, M,
+                     Mq, Mexp, n);
   return hipGetLastError();
 }
 
@@ -1393,8 +1446,8 @@ hipError_t launch_update(float* D, float* Dlo, const float* XT, int64_t n, const
 hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
                                SelState* st, int mult, int* seeds, int64_t npf,
                                const RowList& rl, const float* tauw, const float* frange, int d,
-                               const int* cand, const int* iperm, const unsigned short* M16,
-                               hipStream_t s, int defer) {
+                               const int* cand, const int* iperm, const unsigned char* Mq,
+                               const int* Mexp, hipStream_t s, int defer) {
   // backward lists with warm bounds need no new seeds (k_update_seeds)
   const bool no_seeds = mult < 0 && rl.idx && tauw && !Dlo;
   if (n % 4 || defer < 0 || defer > 2 || (defer && !no_seeds)) return hipErrorInvalidValue;
@@ -1402,19 +1455,19 @@ hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
   const int g = (int)(rows < 8192 ? rows : 8192);
   if (Dlo)
     hipLaunchKernelGGL((k_update_seeds<true, 0, true>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n,
-                       st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
+                       st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, Mq, Mexp);
   else if (defer == 1)
     hipLaunchKernelGGL((k_update_seeds<false, 1, false>), dim3(g), dim3(256), 0, s, D, Dlo, XT,
-                       n, st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
+                       n, st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, Mq, Mexp);
   else if (defer == 2)
     hipLaunchKernelGGL((k_update_seeds<false, 2, false>), dim3(g), dim3(256), 0, s, D, Dlo, XT,
-                       n, st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
+                       n, st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, Mq, Mexp);
   else if (no_seeds)
     hipLaunchKernelGGL((k_update_seeds<false, 0, false>), dim3(g), dim3(256), 0, s, D, Dlo, XT,
-                       n, st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
+                       n, st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, Mq, Mexp);
   else
     hipLaunchKernelGGL((k_update_seeds<false, 0, true>), dim3(g), dim3(256), 0, s, D, Dlo, XT,
-                       n, st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, M16);
+                       n, st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, Mq, Mexp);
   return hipGetLastError();
 }
 
