@@ -128,3 +128,52 @@ def test_forced_kernel_on_workload_recipes_matches_oracle(engine, monkeypatch, p
     steps = g["steps"]
     np.testing.assert_array_equal(g["candidate_errors"][:steps].view(np.uint32),
                                   candidate_error_bits(rec))
+
+
+_NO_DEFER_SCRIPT = r"""
+import json, os, sys
+import numpy as np
+sys.path.insert(0, os.environ["SNN_ROOT"])
+sys.path.insert(0, os.path.join(os.environ["SNN_ROOT"], "tests"))
+from conftest import load_case
+from speedynn_amd import _lib
+rec, X, y = load_case(os.environ["SNN_CASE"])
+inp = rec["inputs"]
+eng = _lib.Engine(0)
+g = eng.select(X, y, inp["k_vals"], [0], method=0, model_type=0, k_fold=inp["k_fold"],
+               num_features=inp["num_features"], form=1, candidate_errors=True)
+steps = int(g["steps"])
+print(json.dumps({"ordering": np.asarray(g["selected_ordering"]).tolist(),
+                  "errors": np.asarray(g["selected_errors"], np.float32).view(np.uint32).tolist(),
+                  "cand": g["candidate_errors"][:steps].view(np.uint32).ravel().tolist()}))
+"""
+
+
+def test_backward_without_deferred_writes_matches_oracle():
+    """SPEEDYNN_DEFER=off (read once per process, so in a process of its own):
+    the backward selection writes D after every step instead of every other,
+    on the n=32768 backward recipe at reduced size; every candidate's error at
+    every step against the oracle fixture, as the deferred default is checked
+    above."""
+    import json
+    import subprocess
+    import sys
+
+    path = REDUCED[1]
+    rec, _, _ = load_case(path)
+    inp = rec["inputs"]
+    assert inp["method"] == "backward" and inp["loss_functions"] == ["rmse"]
+    assert inp["model_type"] == "regression" and inp["n"] % 4 == 0  # row lists, deferred writes
+    env = dict(os.environ, SPEEDYNN_DEFER="off", SNN_CASE=path,
+               SNN_ROOT=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    print(f"subprocess: SPEEDYNN_DEFER=off {rec['name']}", flush=True)
+    out = subprocess.run([sys.executable, "-c", _NO_DEFER_SCRIPT], env=env, capture_output=True,
+                         text=True, timeout=240)
+    assert out.returncode == 0, out.stderr[-2000:]
+    got = json.loads(out.stdout.strip().splitlines()[-1])
+    exp = expected_arrays(rec)
+    np.testing.assert_array_equal(got["ordering"], exp["selected_ordering"])
+    np.testing.assert_array_equal(np.array(got["errors"], np.uint32),
+                                  np.asarray(exp["selected_errors"], np.float32).view(np.uint32))
+    bits = candidate_error_bits(rec)
+    np.testing.assert_array_equal(np.array(got["cand"], np.uint32).reshape(bits.shape), bits)
