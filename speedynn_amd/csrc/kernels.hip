@@ -310,7 +310,11 @@ __global__ __launch_bounds__(256) void k_update(float* __restrict__ D, float* __
         if (COMP) *reinterpret_cast<v4f*>(rlo + j) = lv;
       }
     } else {
-      for (int64_t j = threadIdx.x; j < n; j += 256) {
+      // n % 4 != 0: rows start at any 4-byte boundary.  A scalar head up to
+      // the row's first 16-byte boundary, float4 over the body (D, and Dlo,
+      // which shares the layout), a scalar tail; the feature's values are
+      // read one by one (its column has an alignment of its own)
+      auto one = [&](int64_t j) {
         const float xj = xf[j];
         const float t = xi - xj;
         const float p = t * t;
@@ -319,7 +323,29 @@ __global__ __launch_bounds__(256) void k_update(float* __restrict__ D, float* __
         const float s = dv + ps;
         if (COMP) rlo[j] += comp_error(xi, xj, t, p, dv, ps, s, mult < 0);
         row[j] = s;
+      };
+      const int64_t mis = (i * n) & 3;
+      const int64_t head = (mis ? 4 - mis : 0) < n ? (mis ? 4 - mis : 0) : n;
+      const int64_t end = head + (n - head) / 4 * 4;  // [head, end): whole float4
+      if ((int64_t)threadIdx.x < head) one(threadIdx.x);
+      for (int64_t j = head + (int64_t)threadIdx.x * 4; j < end; j += 1024) {
+        v4f dv = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j));
+        v4f lv = {0.f, 0.f, 0.f, 0.f};
+        if (COMP) lv = *reinterpret_cast<v4f*>(rlo + j);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float xj = xf[j + e];
+          const float t = xi - xj;
+          const float p = t * t;
+          const float ps = mult > 0 ? p : -p;
+          const float sv = dv[e] + ps;
+          if (COMP) lv[e] += comp_error(xi, xj, t, p, dv[e], ps, sv, mult < 0);
+          dv[e] = sv;
+        }
+        __builtin_nontemporal_store(dv, reinterpret_cast<v4f*>(row + j));
+        if (COMP) *reinterpret_cast<v4f*>(rlo + j) = lv;
       }
+      if (end + (int64_t)threadIdx.x < n) one(end + threadIdx.x);
     }
   }
 }
