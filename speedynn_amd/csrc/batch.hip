@@ -38,7 +38,7 @@ struct Slot {
   bool sorted;       // forward, precision form: sorted columns for the first step
   size_t sort_bytes;
   bool comp;         // keep D's compensation term
-  bool fused;        // precision form, n % 4 == 0: the update writes the next step's seeds
+  bool fused;        // precision form: the update writes the next step's seeds
   bool on_device;    // the caller's X and y are device pointers
   const float* dX;   // X and y on the device (the caller's, or copies in the arena)
   const float* dy;
@@ -283,7 +283,7 @@ extern "C" int snn_select_batch(snn_ctx* ctx, int num_problems, snn_problem* pro
     s.perm = take(off, sizeof(int) * d);
     s.key = take(off, sizeof(double) * d);
     s.seeds = pruned ? take(off, sizeof(int) * n * 128) : 0;
-    s.fused = pruned && pb.n % 4 == 0;
+    s.fused = pruned;
     const snn_problem& q = probs[s.index];
     s.has_med = snn::needs_median_scratch(q.loss_functions, q.num_loss);
     s.loss_paths = snn::loss_paths(q.loss_functions, q.num_loss);

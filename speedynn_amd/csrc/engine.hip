@@ -632,9 +632,9 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
   // backward: M is recomputed over the remaining candidates each time their
   // number has halved (d/2, d/4, ...), keeping the bound fl(D - M) tight
   int m_refresh = d / 2;
-  // precision form: each update also writes the next step's seeds (rows of a
-  // multiple of 4 floats); the first pruned step of a backward run scans them
-  const bool fused_seeds = pruned && n % 4 == 0;
+  // precision form: each update also writes the next step's seeds; the first
+  // pruned step of a backward run scans them
+  const bool fused_seeds = pruned;
   // graph mode (params.use_graph; not with a caller's arena): a step's
   // launches are captured once per distinct sequence and replayed; a step
   // whose pruned kernel is still to be tuned runs on the stream.  Graph mode

@@ -15,7 +15,8 @@ hipError_t launch_gram_exact(const float* XT, float* D, float* Dlo, float* M, in
 hipError_t launch_max_square(const float* Xc, const SelState* st, float* M, int64_t n, int cpad,
                              hipStream_t s);
 // The update of the device state's pending feature fused with the seed scan
-// of the next step (seeds as launch_seed_scan writes them); n % 4 == 0.
+// of the next step (seeds as launch_seed_scan writes them); with n % 4 != 0
+// only that (rl.idx null, defer 0).
 // rl.idx set also writes the next search's row lists (RowList) from the warm
 // bounds tauw (forward: and the chosen column's range frange; backward: the
 // lower bounds fl(D - Mq) from Mq = M rounded up to 8 bits, launch_quantize_m).

@@ -696,6 +696,69 @@ void k_update_seeds(float* __restrict__ D,
   }
 }
 
+// The update fused with the next step's seed scan for rows at any 4-byte
+// boundary (n % 4 != 0; no row lists): one wave per row, a scalar head up to
+// the row's first 16-byte boundary, float4 over the body, a scalar tail, each
+// new D_ij offered to the lane's two-smallest pair as k_update_seeds offers
+// it; the row's 128 seeds are written ranked by D.
+template <bool COMP>
+__global__ __launch_bounds__(256) void k_update_seeds_any(float* __restrict__ D,
+                                                          float* __restrict__ Dlo,
+                                                          const float* __restrict__ XT,
+                                                          int64_t n,
+                                                          const SelState* __restrict__ st,
+                                                          int mult, int* __restrict__ seeds,
+                                                          int64_t npf) {
+  if (!st->pending) return;
+  const int lane = threadIdx.x & 63;
+  const float* __restrict__ xf = XT + (int64_t)st->fstar * n;
+  for (int64_t i = (int64_t)blockIdx.x * 4 + uniform(threadIdx.x >> 6); i < n;
+       i += (int64_t)gridDim.x * 4) {
+    const float xi = xf[i];
+    float* __restrict__ row = D + i * n;
+    float* __restrict__ rlo = COMP ? Dlo + i * n : nullptr;
+    const int64_t flo = (i / npf) * npf;
+    const int64_t fhi = flo + npf < n ? flo + npf : n;
+    SeedPair sp;
+    sp.reset();
+    auto term = [&](float xj, float dv, float& lo) {
+      const float t = xi - xj;
+      const float p = t * t;
+      const float ps = mult > 0 ? p : -p;
+      const float sv = dv + ps;
+      if (COMP) lo += comp_error(xi, xj, t, p, dv, ps, sv, mult < 0);
+      return sv;
+    };
+    auto one = [&](int64_t j) {
+      float lo = COMP ? rlo[j] : 0.0f;
+      const float sv = term(xf[j], row[j], lo);
+      row[j] = sv;
+      if (COMP) rlo[j] = lo;
+      sp.offer(sv, j, flo, fhi);
+    };
+    const int64_t mis = (i * n) & 3;
+    const int64_t head = (mis ? 4 - mis : 0) < n ? (mis ? 4 - mis : 0) : n;
+    const int64_t end = head + (n - head) / 4 * 4;  // [head, end): whole float4
+    if (lane < head) one(lane);
+    for (int64_t j = head + (int64_t)lane * 4; j < end; j += 256) {
+      v4f dv = __builtin_nontemporal_load(reinterpret_cast<v4f*>(row + j));
+      v4f lv = {0.f, 0.f, 0.f, 0.f};
+      if (COMP) lv = *reinterpret_cast<v4f*>(rlo + j);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        float lo = lv[e];
+        dv[e] = term(xf[j + e], dv[e], lo);
+        lv[e] = lo;
+        sp.offer(dv[e], j + e, flo, fhi);
+      }
+      __builtin_nontemporal_store(dv, reinterpret_cast<v4f*>(row + j));
+      if (COMP) *reinterpret_cast<v4f*>(rlo + j) = lv;
+    }
+    if (end + lane < n) one(end + lane);
+    sp.write_ranked(seeds + i * 128);
+  }
+}
+
 // ------------------------------------------------------- candidate gather
 
 // Xc[j][c] = X[j][cand[c]] for c < ncand, 0 for the padding slots.
@@ -1474,11 +1537,21 @@ hipError_t launch_update_seeds(float* D, float* Dlo, const float* XT, int64_t n,
                                const RowList& rl, const float* tauw, const float* frange, int d,
                                const int* cand, const int* iperm, const unsigned char* Mq,
                                const int* Mexp, hipStream_t s, int defer) {
-  // backward lists with warm bounds need no new seeds (k_update_seeds)
-  const bool no_seeds = mult < 0 && rl.idx && tauw && !Dlo;
-  if (n % 4 || defer < 0 || defer > 2 || (defer && !no_seeds)) return hipErrorInvalidValue;
   const int64_t rows = (n + 3) / 4;
   const int g = (int)(rows < 8192 ? rows : 8192);
+  if (n % 4) {  // rows off 16-byte boundaries: the seeds only (no lists, no deferred writes)
+    if (rl.idx || defer) return hipErrorInvalidValue;
+    if (Dlo)
+      hipLaunchKernelGGL((k_update_seeds_any<true>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st,
+                         mult, seeds, npf);
+    else
+      hipLaunchKernelGGL((k_update_seeds_any<false>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n, st,
+                         mult, seeds, npf);
+    return hipGetLastError();
+  }
+  // backward lists with warm bounds need no new seeds (k_update_seeds)
+  const bool no_seeds = mult < 0 && rl.idx && tauw && !Dlo;
+  if (defer < 0 || defer > 2 || (defer && !no_seeds)) return hipErrorInvalidValue;
   if (Dlo)
     hipLaunchKernelGGL((k_update_seeds<true, 0, true>), dim3(g), dim3(256), 0, s, D, Dlo, XT, n,
                        st, mult, seeds, npf, rl, tauw, frange, d, cand, iperm, Mq, Mexp);
