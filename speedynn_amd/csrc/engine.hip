@@ -157,10 +157,10 @@ void destroy_events(snn_ctx* ctx) {
 // per (query, W candidate blocks) sharing the row scan (ceil(blocks / W)
 // workgroups per query) -- depends on the data (how much the candidate
 // blocks' bounds differ: a workgroup waits for its loosest block) and on the
-// step (early steps keep many points, late ones few), not only on the shape.  The first step of a
-// shape (n, d, list size, direction) and step range [2^b, 2^(b+1)) runs every
-// applicable kernel on the same step, timed with HIP events, and the fastest
-// serves the rest of that range.  With nc kernels to try, kernel r serves the
+// step (early steps keep many points, late ones few), not only on the shape.
+// The first step of a shape (n, d, list size, direction) and step range
+// [2^b, 2^(b+1)) runs every applicable kernel on the same step, timed with
+// HIP events, and the fastest serves the rest of that range.  With nc kernels to try, kernel r serves the
 // queries r, r + nc, r + 2 nc, ... of that step (the order of the kernels
 // rotates with the range, so no kernel always pays for cold caches): the
 // parts are disjoint and cover every query once, and every kernel gives the
