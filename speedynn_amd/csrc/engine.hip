@@ -175,6 +175,13 @@ void destroy_events(snn_ctx* ctx) {
 std::mutex g_tune_mu;
 using TuneKey = std::tuple<int64_t, int, int, int, int>;  // n, d, list size, mult, range
 std::map<TuneKey, int> g_tuned;
+// Row-list capacity per shape (n, d, list size, mult): n / divisor entries per
+// row (default 8).  A longer cap keeps lists for rows whose list would overflow
+// (the search then scans the list, not the row) at the price of writing them;
+// which wins depends on the data, so it is a recorded choice, measured per
+// bench shape, not tuned online.
+using ShapeKey = std::tuple<int64_t, int, int, int>;
+std::map<ShapeKey, int> g_listcap;
 bool g_tune_loaded = false;
 std::string g_tune_write;  // file rewritten after new choices (empty: none)
 
@@ -204,6 +211,11 @@ void load_tuned_locked() {
                     &n, &d, &K, &mult, &b, &v) == 6 &&
         v >= 1 && v <= 7)
       g_tuned[std::make_tuple((int64_t)n, d, K, mult, b)] = v;
+    else if (std::sscanf(line, " { \"n\": %lld, \"d\": %d, \"list\": %d, \"mult\": %d, "
+                               "\"list_cap_div\": %d",
+                         &n, &d, &K, &mult, &v) == 5 &&
+             v >= 2 && v <= 64)
+      g_listcap[std::make_tuple((int64_t)n, d, K, mult)] = v;
   }
   std::fclose(f);
 }
@@ -215,16 +227,44 @@ void save_tuned_locked() {
   if (!f) return;
   std::fprintf(f, "[\n");
   size_t i = 0;
+  const size_t total = g_tuned.size() + g_listcap.size();
   for (const auto& kv : g_tuned) {
     const TuneKey& k = kv.first;
     std::fprintf(f, "  {\"n\": %lld, \"d\": %d, \"list\": %d, \"mult\": %d, \"range\": %d, "
                     "\"kernel\": %d}%s\n",
                  (long long)std::get<0>(k), std::get<1>(k), std::get<2>(k), std::get<3>(k),
-                 std::get<4>(k), kv.second, ++i < g_tuned.size() ? "," : "");
+                 std::get<4>(k), kv.second, ++i < total ? "," : "");
+  }
+  for (const auto& kv : g_listcap) {
+    const ShapeKey& k = kv.first;
+    std::fprintf(f, "  {\"n\": %lld, \"d\": %d, \"list\": %d, \"mult\": %d, "
+                    "\"list_cap_div\": %d}%s\n",
+                 (long long)std::get<0>(k), std::get<1>(k), std::get<2>(k), std::get<3>(k),
+                 kv.second, ++i < total ? "," : "");
   }
   std::fprintf(f, "]\n");
   std::fclose(f);
   std::rename(tmp.c_str(), g_tune_write.c_str());
+}
+
+// Entries per row list for a shape: n / divisor (SPEEDYNN_LIST_CAP_DIV, else a
+// recorded choice, else 8), at least 256, at most n.
+int64_t list_cap(int64_t n, int d, int K, int mult) {
+  static const int forced = [] {
+    const char* e = std::getenv("SPEEDYNN_LIST_CAP_DIV");
+    const int v = e ? std::atoi(e) : 0;
+    return v >= 2 && v <= 64 ? v : 0;
+  }();
+  int div = 8;
+  if (forced) {
+    div = forced;
+  } else {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    load_tuned_locked();
+    auto it = g_listcap.find(std::make_tuple(n, d, K, mult));
+    if (it != g_listcap.end()) div = it->second;
+  }
+  return std::min<int64_t>(n, std::max<int64_t>(256, n / div));
 }
 
 // Variant 4: the walk along the first chosen feature's sorted column, for a
@@ -517,9 +557,9 @@ int select_impl(snn_ctx* ctx, const float* dX, int64_t n, int d, const float* dy
     bd.add(ctx->frange, sizeof(float) * 2, &w.frange);
   }
   // precision form: the fused update writes each row's list of points the
-  // next search may visit (up to n/8 entries, at least 256, per row)
+  // next search may visit (list_cap: n/8 entries per row unless recorded otherwise)
   const int64_t rows_cap =
-      (pruned && n % 4 == 0) ? std::min<int64_t>(n, std::max<int64_t>(256, n / 8)) : 0;
+      (pruned && n % 4 == 0) ? list_cap(n, d, snn::search_list_size(kmax), fwd ? 1 : -1) : 0;
   if (rows_cap) {
     bd.add(ctx->lidx, sizeof(int) * (size_t)n * rows_cap, &w.lidx);
     bd.add(ctx->lval, sizeof(float) * (size_t)n * rows_cap, &w.lval);

@@ -177,3 +177,56 @@ def test_backward_without_deferred_writes_matches_oracle():
                                   np.asarray(exp["selected_errors"], np.float32).view(np.uint32))
     bits = candidate_error_bits(rec)
     np.testing.assert_array_equal(np.array(got["cand"], np.uint32).reshape(bits.shape), bits)
+
+
+_TUNE_FILE_SCRIPT = r"""
+import json, os, sys
+import numpy as np
+sys.path.insert(0, os.environ["SNN_ROOT"])
+sys.path.insert(0, os.path.join(os.environ["SNN_ROOT"], "tests"))
+from conftest import load_case
+from speedynn_amd import _lib
+rec, X, y = load_case(os.environ["SNN_CASE"])
+inp = rec["inputs"]
+eng = _lib.Engine(0)
+g = eng.select(X, y, inp["k_vals"], [4], method=1, model_type=1, k_fold=inp["k_fold"],
+               num_features=inp["num_features"], form=1, candidate_errors=True)
+steps = int(g["steps"])
+print(json.dumps({"ordering": np.asarray(g["selected_ordering"]).tolist(),
+                  "cand": g["candidate_errors"][:steps].view(np.uint32).ravel().tolist()}))
+"""
+
+
+def test_tune_file_list_cap_and_choices_round_trip(tmp_path):
+    """A tune file (SPEEDYNN_TUNE_FILE, read once per process, so in a process
+    of its own) that records a row-list cap of n/2 for the n=65536 forward
+    recipe's reduced shape: the selection with those longer lists equals the
+    oracle fixture in every candidate error, and the file is rewritten with
+    the cap kept beside the kernel choices the tuning trials made."""
+    import json
+    import subprocess
+    import sys
+
+    path = REDUCED[0]
+    rec, _, _ = load_case(path)
+    inp = rec["inputs"]
+    assert inp["method"] == "forward" and inp["n"] % 4 == 0 and inp["k_vals"] == [20]
+    assert inp["loss_functions"] == ["zero_one"] and inp["model_type"] == "classification"
+    tune = tmp_path / "tune.json"
+    cap_line = {"n": inp["n"], "d": inp["d"], "list": 20, "mult": 1, "list_cap_div": 2}
+    tune.write_text("[\n  " + json.dumps(cap_line) + "\n]\n")
+    env = dict(os.environ, SPEEDYNN_TUNE_FILE=str(tune), SNN_CASE=path,
+               SNN_ROOT=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    env.pop("SPEEDYNN_TUNING", None)
+    env.pop("SPEEDYNN_PRUNED", None)
+    print(f"subprocess: tune file with list_cap_div 2, {rec['name']}", flush=True)
+    out = subprocess.run([sys.executable, "-c", _TUNE_FILE_SCRIPT], env=env, capture_output=True,
+                         text=True, timeout=240)
+    assert out.returncode == 0, out.stderr[-2000:]
+    got = json.loads(out.stdout.strip().splitlines()[-1])
+    np.testing.assert_array_equal(got["ordering"], expected_arrays(rec)["selected_ordering"])
+    bits = candidate_error_bits(rec)
+    np.testing.assert_array_equal(np.array(got["cand"], np.uint32).reshape(bits.shape), bits)
+    entries = json.loads(tune.read_text())
+    assert cap_line in entries
+    assert any(e.get("kernel") for e in entries if e.get("n") == inp["n"])
